@@ -1,0 +1,140 @@
+"""ctypes binding of the C ABI in include/alsvinn_hip.h.
+
+The shared library is loaded from the package directory only (built in-tree
+by alsvinn_amd/build.py).  There is no fallback: if the library is missing or
+cannot be loaded, every entry point raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libalsvinn_hip.so")
+
+# Exported entry points, in the order include/alsvinn_hip.h declares them.
+SYMBOLS = [
+    "avh_last_error",
+    "avh_ctx_create",
+    "avh_ctx_destroy",
+    "avh_solver_create",
+    "avh_solver_destroy",
+    "avh_ghost_cells",
+    "avh_num_vars",
+    "avh_set_state",
+    "avh_get_state",
+    "avh_set_state_device",
+    "avh_get_state_device",
+    "avh_set_time",
+    "avh_add_fixed_interval_adjuster",
+    "avh_step",
+    "avh_advance",
+    "avh_ack_saves",
+    "avh_get_progress",
+    "avh_set_end_time",
+    "avh_enqueue_steps",
+    "avh_synchronize",
+    "avh_time_step_kernels",
+    "avh_stream",
+    "avh_compute_flux",
+]
+
+AVH_OK, AVH_EINVAL, AVH_EHIP, AVH_ECONSTRAINT, AVH_ENOMEM = range(5)
+
+
+class AvhConfig(ctypes.Structure):
+    _fields_ = [
+        ("equation", ctypes.c_int),
+        ("reconstruction", ctypes.c_int),
+        ("flux", ctypes.c_int),
+        ("integrator", ctypes.c_int),
+        ("boundary", ctypes.c_int),
+        ("n", ctypes.c_int * 3),
+        ("lower", ctypes.c_double * 3),
+        ("upper", ctypes.c_double * 3),
+        ("gamma", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+    ]
+
+
+class AvhProgress(ctypes.Structure):
+    _fields_ = [
+        ("time", ctypes.c_double),
+        ("last_dt", ctypes.c_double),
+        ("steps", ctypes.c_int64),
+        ("violated", ctypes.c_int32),
+        ("saves_pending", ctypes.c_int32),
+    ]
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libalsvinn_hip.so and declare every prototype.  Raises
+    LibraryMissing if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(
+            "%s is not built; run alsvinn_amd/build.py (or __graft_entry__.build())" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    P = ctypes.POINTER
+    d = ctypes.c_double
+    i64 = ctypes.c_int64
+    st = ctypes.c_int
+    L.avh_last_error.restype = ctypes.c_char_p
+    L.avh_last_error.argtypes = []
+    protos = {
+        "avh_ctx_create": [ctypes.c_int, P(vp)],
+        "avh_ctx_destroy": [vp],
+        "avh_solver_create": [vp, P(AvhConfig), P(vp)],
+        "avh_solver_destroy": [vp],
+        "avh_ghost_cells": [vp, P(ctypes.c_int)],
+        "avh_num_vars": [vp, P(ctypes.c_int)],
+        "avh_set_state": [vp, vp],
+        "avh_get_state": [vp, vp],
+        "avh_set_state_device": [vp, vp],
+        "avh_get_state_device": [vp, vp],
+        "avh_set_time": [vp, d, i64],
+        "avh_add_fixed_interval_adjuster": [vp, d, d, ctypes.c_int],
+        "avh_step": [vp, P(d)],
+        "avh_advance": [vp, d, i64, ctypes.c_int, ctypes.c_int, P(AvhProgress)],
+        "avh_ack_saves": [vp],
+        "avh_get_progress": [vp, P(AvhProgress)],
+        "avh_set_end_time": [vp, d],
+        "avh_enqueue_steps": [vp, i64],
+        "avh_synchronize": [vp],
+        "avh_compute_flux": [vp, vp, P(d)],
+        "avh_time_step_kernels": [vp, i64, P(d), P(ctypes.c_int)],
+    }
+    for name, args in protos.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = st
+    L.avh_stream.argtypes = [vp]
+    L.avh_stream.restype = vp
+    _lib = L
+    return L
+
+
+class HipError(RuntimeError):
+    """Raised for AVH_EHIP / AVH_ENOMEM / AVH_EINVAL."""
+
+
+class ConstraintError(RuntimeError):
+    """AVH_ECONSTRAINT: the reference throws std::runtime_error from
+    Simulator::checkConstraints (alsfvm/src/simulator/Simulator.cpp:180-189)."""
+
+
+def check(status):
+    if status == AVH_OK:
+        return
+    msg = load().avh_last_error().decode(errors="replace")
+    if status == AVH_ECONSTRAINT:
+        raise ConstraintError(msg)
+    raise HipError("avh status %d: %s" % (status, msg))

@@ -1,0 +1,628 @@
+// C ABI of the hip platform (include/alsvinn_hip.h): solver objects, device
+// buffers, the per-step kernel sequence of each integrator, the ghost-filled
+// state export and the small element-wise kernels.
+//
+// Buffers (interior-only SoA, nvars * ncells doubles each):
+//   A  the state U0 (always holds the current solution between steps)
+//   B  L0 = unscaled net flux of stage 1
+//   C  U1 (rungekutta2) or U2 (rungekutta3)
+// Step sequences (reference: alsfvm/src/integrator/{ForwardEuler,RungeKutta2,
+// RungeKutta3}.cpp, Simulator::incrementSolution Simulator.cpp:191-214):
+//   forwardeuler: stage(L: A->B) ; ew(A = B*s + A, check, end step)
+//   rungekutta2 : stage(L: A->B) ; ew(C = B*s + A) ; stage(RK2_2: C, A -> A)
+//   rungekutta3 : stage(L: A->B) ; stage(RK3_2: B*s+A on the fly -> C) ;
+//                 stage(RK3_3: C, A -> A)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/alsvinn_hip.h"
+#include "avh_stage_inst.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace avh;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+avh_status fail(avh_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+#define AVH_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(AVH_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+int nvars_of(int eq) {
+    switch (eq) {
+    case AVH_EQ_BURGERS: return 1;
+    case AVH_EQ_EULER1: return 3;
+    case AVH_EQ_EULER2: return 4;
+    case AVH_EQ_EULER3: return 5;
+    }
+    return 0;
+}
+
+int ghosts_of(int recon) {
+    switch (recon) {
+    case AVH_RECON_NONE: return 1;
+    case AVH_RECON_WENO2: return 2;
+    case AVH_RECON_ENO2: return 2;
+    case AVH_RECON_ENO3: return 3;
+    }
+    return 0;
+}
+
+enum EwMode { EW_FE_FINISH = 0, EW_RK2_U1 = 1 };
+
+// out = L*s + U0 element-wise (Volume *= dt/dx; += U0).  EW_FE_FINISH writes
+// in place into U0, checks the constraints and closes the step.
+template <class EQ>
+__global__ __launch_bounds__(256) void ew_kernel(const double* L, const double* u0, double* out,
+                                                 Control* ctrl, long long nc, int mode, double gamma) {
+    __shared__ int s_flag;
+    if (!ctrl->active) return;
+    constexpr int NV = EQ::NV;
+    const double s = ctrl->scale;
+    bool bad = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        double r[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double o = L[v * nc + i] * s;
+            o = o + u0[v * nc + i];
+            r[v] = o;
+        }
+        if (mode == EW_FE_FINISH) bad = bad || !EQ::obeys(r, gamma);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out[v * nc + i] = r[v];
+    }
+    if (mode == EW_FE_FINISH) {
+        const bool any_bad = __syncthreads_or(bad);
+        if (any_bad && threadIdx.x == 0) atomicOr(&ctrl->block_violation, 1);
+        if (arrive_last(ctrl, gridDim.x, &s_flag) && threadIdx.x == 0) finish_step(ctrl);
+    }
+}
+
+// Ghosted export: every cell of the (n+2g)-sized volume takes the value of
+// the interior cell the boundary map assigns to it.
+__global__ __launch_bounds__(256) void ghost_fill_kernel(const double* in, double* out, int nv,
+                                                         int n0, int n1, int n2, int g0, int g1,
+                                                         int g2, int bc) {
+    const long long t0 = n0 + 2 * g0, t1 = n1 + 2 * g1, t2 = n2 + 2 * g2;
+    const long long total = t0 * t1 * t2;
+    const long long nc = (long long)n0 * n1 * n2;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % t0);
+        const int y = (int)((i / t0) % t1);
+        const int z = (int)(i / (t0 * t1));
+        const int sx = bc_map(x - g0, n0, bc);
+        const int sy = g1 ? bc_map(y - g1, n1, bc) : 0;
+        const int sz = g2 ? bc_map(z - g2, n2, bc) : 0;
+        const long long si = ((long long)sz * n1 + sy) * n0 + sx;
+        for (int v = 0; v < nv; ++v) out[v * total + i] = in[v * nc + si];
+    }
+}
+
+typedef void (*EwLauncher)(const double*, const double*, double*, Control*, long long, int, double,
+                           hipStream_t);
+
+template <class EQ>
+void launch_ew(const double* L, const double* u0, double* out, Control* c, long long nc, int mode,
+               double gamma, hipStream_t st) {
+    long long nb = (nc + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(ew_kernel<EQ>, dim3((unsigned)nb), dim3(256), 0, st, L, u0, out, c, nc, mode,
+                       gamma);
+}
+
+EwLauncher ew_for(int eq) {
+    switch (eq) {
+    case AVH_EQ_BURGERS: return &launch_ew<BurgersEq>;
+    case AVH_EQ_EULER1: return &launch_ew<EulerEq<1>>;
+    case AVH_EQ_EULER2: return &launch_ew<EulerEq<2>>;
+    case AVH_EQ_EULER3: return &launch_ew<EulerEq<3>>;
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+struct avh_ctx {
+    int device;
+};
+
+struct avh_solver {
+    avh_ctx* ctx = nullptr;
+    avh_config cfg{};
+    int dim = 1, nv = 0, ng = 0;
+    int g[3] = {0, 0, 0};
+    long long ncells = 0;
+    long long nghosted = 0;
+    double *A = nullptr, *B = nullptr, *C = nullptr, *staging = nullptr;
+    Control* ctrl = nullptr;
+    Control h{};  // host mirror of ctrl, valid after pull()
+    hipStream_t stream = nullptr;
+    StageLauncher stage = nullptr;
+    EwLauncher ew = nullptr;
+};
+
+static avh_status use_device(const avh_solver* s) {
+    AVH_HIP(hipSetDevice(s->ctx->device));
+    return AVH_OK;
+}
+
+static avh_status pull(avh_solver* s) {
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    AVH_HIP(hipMemcpy(&s->h, s->ctrl, sizeof(Control), hipMemcpyDeviceToHost));
+    return AVH_OK;
+}
+
+static avh_status push(avh_solver* s) {
+    AVH_HIP(hipMemcpy(s->ctrl, &s->h, sizeof(Control), hipMemcpyHostToDevice));
+    return AVH_OK;
+}
+
+static void recompute_active(Control& h) {
+    h.active = (!(h.t >= h.t_end) && !h.violated && !(h.pause_on_save && h.saves_pending > 0)) ? 1 : 0;
+}
+
+// Optional per-launch event timing (avh_time_step_kernels).
+struct KernelTimer {
+    hipEvent_t ev[8];
+    int count = 0;
+};
+
+static avh_status mark(avh_solver* s, KernelTimer* t) {
+    if (!t) return AVH_OK;
+    AVH_HIP(hipEventRecord(t->ev[t->count++], s->stream));
+    return AVH_OK;
+}
+
+static avh_status enqueue_step(avh_solver* s, KernelTimer* tm = nullptr) {
+    StageParams P;
+    P.ctrl = s->ctrl;
+    P.ncells = s->ncells;
+    for (int d = 0; d < 3; ++d) P.n[d] = s->cfg.n[d];
+    P.bc = s->cfg.boundary;
+    P.gamma = s->cfg.gamma;
+    // stage 1: L0 = L(U0) into B, dt from the wave speeds
+    P.in = s->A;
+    P.u0 = s->A;
+    P.out = s->B;
+    P.mode = MODE_L;
+    avh_status r;
+    if ((r = mark(s, tm))) return r;
+    AVH_HIP(s->stage(P, s->stream));
+    if ((r = mark(s, tm))) return r;
+    if (s->cfg.integrator == AVH_INT_FORWARDEULER) {
+        s->ew(s->B, s->A, s->A, s->ctrl, s->ncells, EW_FE_FINISH, s->cfg.gamma, s->stream);
+        AVH_HIP(hipGetLastError());
+        if ((r = mark(s, tm))) return r;
+    } else if (s->cfg.integrator == AVH_INT_RUNGEKUTTA2) {
+        s->ew(s->B, s->A, s->C, s->ctrl, s->ncells, EW_RK2_U1, s->cfg.gamma, s->stream);
+        AVH_HIP(hipGetLastError());
+        if ((r = mark(s, tm))) return r;
+        P.in = s->C;
+        P.u0 = s->A;
+        P.out = s->A;
+        P.mode = MODE_RK2_2;
+        AVH_HIP(s->stage(P, s->stream));
+        if ((r = mark(s, tm))) return r;
+    } else {
+        P.in = s->B;
+        P.u0 = s->A;
+        P.out = s->C;
+        P.mode = MODE_RK3_2;
+        AVH_HIP(s->stage(P, s->stream));
+        if ((r = mark(s, tm))) return r;
+        P.in = s->C;
+        P.u0 = s->A;
+        P.out = s->A;
+        P.mode = MODE_RK3_3;
+        AVH_HIP(s->stage(P, s->stream));
+        if ((r = mark(s, tm))) return r;
+    }
+    return AVH_OK;
+}
+
+extern "C" {
+
+const char* avh_last_error(void) { return g_last_error.c_str(); }
+
+avh_status avh_ctx_create(int device, avh_ctx** ctx) {
+    if (!ctx) return fail(AVH_EINVAL, "ctx is null");
+    int count = 0;
+    AVH_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(AVH_EINVAL, "no such device");
+    AVH_HIP(hipSetDevice(device));
+    auto c = new (std::nothrow) avh_ctx;
+    if (!c) return fail(AVH_ENOMEM, "host allocation failed");
+    c->device = device;
+    *ctx = c;
+    return AVH_OK;
+}
+
+avh_status avh_ctx_destroy(avh_ctx* ctx) {
+    delete ctx;
+    return AVH_OK;
+}
+
+avh_status avh_solver_destroy(avh_solver* s) {
+    if (!s) return AVH_OK;
+    hipSetDevice(s->ctx->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    hipFree(s->A);
+    hipFree(s->B);
+    hipFree(s->C);
+    hipFree(s->staging);
+    hipFree(s->ctrl);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return AVH_OK;
+}
+
+avh_status avh_solver_create(avh_ctx* ctx, const avh_config* cfg, avh_solver** out) {
+    if (!ctx || !cfg || !out) return fail(AVH_EINVAL, "null argument");
+    *out = nullptr;
+    const int nv = nvars_of(cfg->equation);
+    if (!nv) return fail(AVH_EINVAL, "unknown equation");
+    const int ng = ghosts_of(cfg->reconstruction);
+    if (!ng) return fail(AVH_EINVAL, "unknown reconstruction");
+    if (cfg->integrator < AVH_INT_FORWARDEULER || cfg->integrator > AVH_INT_RUNGEKUTTA3)
+        return fail(AVH_EINVAL, "unknown integrator");
+    if (cfg->boundary != AVH_BC_PERIODIC && cfg->boundary != AVH_BC_NEUMANN)
+        return fail(AVH_EINVAL, "unknown boundary");
+    for (int d = 0; d < 3; ++d)
+        if (cfg->n[d] < 1) return fail(AVH_EINVAL, "grid sizes must be >= 1");
+    const int dim = cfg->n[2] > 1 ? 3 : (cfg->n[1] > 1 ? 2 : 1);
+    for (int d = 0; d < dim; ++d)
+        if (cfg->n[d] < ng)
+            return fail(AVH_EINVAL, "too few cells: every active axis needs at least as many cells as ghost cells");
+    StageLauncher st = nullptr;
+    switch (cfg->equation) {
+    case AVH_EQ_BURGERS: st = get_launcher_burgers(cfg->reconstruction, cfg->flux, dim); break;
+    case AVH_EQ_EULER1: st = get_launcher_euler1(cfg->reconstruction, cfg->flux, dim); break;
+    case AVH_EQ_EULER2: st = get_launcher_euler2(cfg->reconstruction, cfg->flux, dim); break;
+    case AVH_EQ_EULER3: st = get_launcher_euler3(cfg->reconstruction, cfg->flux, dim); break;
+    }
+    if (!st)
+        return fail(AVH_EINVAL,
+                    "unsupported combination of equation, flux and grid dimension (burgers: godunov; "
+                    "euler<n>: hll, hll3 with at most n active axes)");
+
+    AVH_HIP(hipSetDevice(ctx->device));
+    auto s = new (std::nothrow) avh_solver;
+    if (!s) return fail(AVH_ENOMEM, "host allocation failed");
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    s->dim = dim;
+    s->nv = nv;
+    s->ng = ng;
+    s->g[0] = ng;
+    s->g[1] = dim > 1 ? ng : 0;
+    s->g[2] = dim > 2 ? ng : 0;
+    s->ncells = (long long)cfg->n[0] * cfg->n[1] * cfg->n[2];
+    s->nghosted = (long long)(cfg->n[0] + 2 * s->g[0]) * (cfg->n[1] + 2 * s->g[1]) * (cfg->n[2] + 2 * s->g[2]);
+    s->stage = st;
+    s->ew = ew_for(cfg->equation);
+    const size_t bytes = (size_t)nv * s->ncells * sizeof(double);
+    if (hipMalloc(&s->A, bytes) != hipSuccess || hipMalloc(&s->B, bytes) != hipSuccess ||
+        hipMalloc(&s->C, bytes) != hipSuccess ||
+        hipMalloc(&s->staging, (size_t)nv * s->nghosted * sizeof(double)) != hipSuccess ||
+        hipMalloc(&s->ctrl, sizeof(Control)) != hipSuccess) {
+        avh_solver_destroy(s);
+        return fail(AVH_ENOMEM, "device allocation failed");
+    }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        s->stream = nullptr;
+        avh_solver_destroy(s);
+        return fail(AVH_EHIP, "hipStreamCreate failed");
+    }
+    hipMemset(s->A, 0, bytes);
+    hipMemset(s->B, 0, bytes);
+    hipMemset(s->C, 0, bytes);
+    // Grid::computeCellLengths: (top - origin)/N, 1 on single-cell axes
+    // (alsfvm/src/grid/Grid.cpp:24-41)
+    Control& h = s->h;
+    std::memset(&h, 0, sizeof(Control));
+    for (int d = 0; d < 3; ++d) h.cell[d] = (cfg->upper[d] - cfg->lower[d]) / double(cfg->n[d]);
+    if (cfg->n[1] == 1) h.cell[1] = 1;
+    if (cfg->n[2] == 1) h.cell[2] = 1;
+    h.cfl = cfg->cfl;
+    h.t_end = std::numeric_limits<double>::infinity();
+    recompute_active(h);
+    avh_status r = push(s);
+    if (r != AVH_OK) {
+        avh_solver_destroy(s);
+        return r;
+    }
+    *out = s;
+    return AVH_OK;
+}
+
+avh_status avh_ghost_cells(const avh_solver* s, int* ghosts) {
+    if (!s || !ghosts) return fail(AVH_EINVAL, "null argument");
+    *ghosts = s->ng;
+    return AVH_OK;
+}
+
+avh_status avh_num_vars(const avh_solver* s, int* nvars) {
+    if (!s || !nvars) return fail(AVH_EINVAL, "null argument");
+    *nvars = s->nv;
+    return AVH_OK;
+}
+
+avh_status avh_set_state(avh_solver* s, const double* host) {
+    if (!s || !host) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    const int n0 = s->cfg.n[0], n1 = s->cfg.n[1], n2 = s->cfg.n[2];
+    const long long t0 = n0 + 2 * s->g[0], t1 = n1 + 2 * s->g[1];
+    std::vector<double> packed((size_t)s->nv * s->ncells);
+    for (int v = 0; v < s->nv; ++v)
+        for (int z = 0; z < n2; ++z)
+            for (int y = 0; y < n1; ++y) {
+                const double* src = host + (size_t)v * s->nghosted +
+                                    ((size_t)(z + s->g[2]) * t1 + (y + s->g[1])) * t0 + s->g[0];
+                double* dst = packed.data() + (size_t)v * s->ncells + ((size_t)z * n1 + y) * n0;
+                std::memcpy(dst, src, (size_t)n0 * sizeof(double));
+            }
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    AVH_HIP(hipMemcpy(s->A, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice));
+    return AVH_OK;
+}
+
+avh_status avh_set_state_device(avh_solver* s, const double* dev) {
+    if (!s || !dev) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    AVH_HIP(hipMemcpyAsync(s->A, dev, (size_t)s->nv * s->ncells * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->stream));
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    return AVH_OK;
+}
+
+avh_status avh_get_state_device(avh_solver* s, double* dev) {
+    if (!s || !dev) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    AVH_HIP(hipMemcpyAsync(dev, s->A, (size_t)s->nv * s->ncells * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->stream));
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    return AVH_OK;
+}
+
+avh_status avh_get_state(avh_solver* s, double* host) {
+    if (!s || !host) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    long long nb = (s->nghosted + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(ghost_fill_kernel, dim3((unsigned)nb), dim3(256), 0, s->stream, s->A, s->staging, s->nv,
+                       s->cfg.n[0], s->cfg.n[1], s->cfg.n[2], s->g[0], s->g[1], s->g[2], s->cfg.boundary);
+    AVH_HIP(hipGetLastError());
+    AVH_HIP(hipMemcpyAsync(host, s->staging, (size_t)s->nv * s->nghosted * sizeof(double),
+                           hipMemcpyDeviceToHost, s->stream));
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    return AVH_OK;
+}
+
+avh_status avh_set_time(avh_solver* s, double time, int64_t steps) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    if ((r = pull(s))) return r;
+    s->h.t = time;
+    s->h.steps = steps;
+    s->h.violated = 0;
+    s->h.saves_pending = 0;
+    recompute_active(s->h);
+    return push(s);
+}
+
+avh_status avh_add_fixed_interval_adjuster(avh_solver* s, double interval, double start, int write_initial) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    if (!(interval > 0)) return fail(AVH_EINVAL, "interval must be positive");
+    avh_status r = use_device(s);
+    if (r) return r;
+    if ((r = pull(s))) return r;
+    if (s->h.n_adj >= MAX_ADJ) return fail(AVH_EINVAL, "at most 4 fixed-interval adjusters");
+    const int i = s->h.n_adj++;
+    s->h.adj_interval[i] = interval;
+    s->h.adj_start[i] = start;
+    s->h.adj_saved[i] = write_initial ? 0 : 1;
+    return push(s);
+}
+
+avh_status avh_set_end_time(avh_solver* s, double t_end) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    if ((r = pull(s))) return r;
+    s->h.t_end = t_end;
+    recompute_active(s->h);
+    return push(s);
+}
+
+avh_status avh_enqueue_steps(avh_solver* s, int64_t steps) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    for (int64_t i = 0; i < steps; ++i)
+        if ((r = enqueue_step(s))) return r;
+    return AVH_OK;
+}
+
+avh_status avh_synchronize(avh_solver* s) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    return AVH_OK;
+}
+
+void* avh_stream(avh_solver* s) { return s ? (void*)s->stream : nullptr; }
+
+avh_status avh_time_step_kernels(avh_solver* s, int64_t steps, double* ms, int* kernels) {
+    if (!s || !ms || !kernels || steps < 1) return fail(AVH_EINVAL, "bad argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    const int nk = s->cfg.integrator == AVH_INT_FORWARDEULER ? 2 : 3;
+    KernelTimer t;
+    for (int i = 0; i <= nk; ++i) AVH_HIP(hipEventCreate(&t.ev[i]));
+    double acc[4] = {0, 0, 0, 0};
+    for (int64_t k = 0; k < steps && r == AVH_OK; ++k) {
+        t.count = 0;
+        r = enqueue_step(s, &t);
+        if (r) break;
+        if (hipEventSynchronize(t.ev[nk]) != hipSuccess) {
+            r = fail(AVH_EHIP, "hipEventSynchronize failed");
+            break;
+        }
+        for (int i = 0; i < nk; ++i) {
+            float f = 0;
+            hipEventElapsedTime(&f, t.ev[i], t.ev[i + 1]);
+            acc[i] += f;
+        }
+    }
+    for (int i = 0; i <= nk; ++i) hipEventDestroy(t.ev[i]);
+    if (r) return r;
+    for (int i = 0; i < nk; ++i) ms[i] = acc[i] / (double)steps;
+    *kernels = nk;
+    return AVH_OK;
+}
+
+static void fill_progress(const avh_solver* s, avh_progress* p) {
+    if (!p) return;
+    p->time = s->h.t;
+    p->last_dt = s->h.last_dt;
+    p->steps = s->h.steps;
+    p->violated = s->h.violated;
+    p->saves_pending = s->h.saves_pending;
+}
+
+avh_status avh_get_progress(avh_solver* s, avh_progress* p) {
+    if (!s || !p) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    if ((r = pull(s))) return r;
+    fill_progress(s, p);
+    return AVH_OK;
+}
+
+avh_status avh_ack_saves(avh_solver* s) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    if ((r = pull(s))) return r;
+    s->h.saves_pending = 0;
+    recompute_active(s->h);
+    return push(s);
+}
+
+static avh_status constraint_error(const avh_solver* s) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf,
+                  "Simulation state does not obey constraints!  At time %.17g, number of timesteps performed %lld",
+                  s->h.t, (long long)s->h.steps);
+    return fail(AVH_ECONSTRAINT, buf);
+}
+
+avh_status avh_step(avh_solver* s, double* dt_out) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    if ((r = pull(s))) return r;
+    if (s->h.violated) return constraint_error(s);
+    // performStep runs regardless of the end time; saves stay counted
+    const double t_end = s->h.t_end;
+    const int pause = s->h.pause_on_save;
+    s->h.t_end = std::numeric_limits<double>::infinity();
+    s->h.pause_on_save = 0;
+    s->h.active = 1;
+    if ((r = push(s))) return r;
+    if ((r = enqueue_step(s))) return r;
+    if ((r = pull(s))) return r;
+    s->h.t_end = t_end;
+    s->h.pause_on_save = pause;
+    recompute_active(s->h);
+    if ((r = push(s))) return r;
+    if (dt_out) *dt_out = s->h.last_dt;
+    if (s->h.violated) return constraint_error(s);
+    return AVH_OK;
+}
+
+avh_status avh_advance(avh_solver* s, double t_end, int64_t max_steps, int chunk, int pause_on_save,
+                       avh_progress* progress) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    if (chunk < 1) chunk = 1;
+    avh_status r = use_device(s);
+    if (r) return r;
+    if ((r = pull(s))) return r;
+    if (s->h.violated) {
+        fill_progress(s, progress);
+        return constraint_error(s);
+    }
+    s->h.t_end = t_end;
+    s->h.pause_on_save = pause_on_save ? 1 : 0;
+    recompute_active(s->h);
+    if ((r = push(s))) return r;
+    int64_t done = 0;
+    while (done < max_steps && s->h.active) {
+        const int64_t k = (max_steps - done) < chunk ? (max_steps - done) : chunk;
+        const int64_t before = s->h.steps;
+        for (int64_t i = 0; i < k; ++i)
+            if ((r = enqueue_step(s))) return r;
+        if ((r = pull(s))) return r;
+        done += s->h.steps - before;
+        if (s->h.steps - before < k) break;  // the device loop ended inside the chunk
+    }
+    fill_progress(s, progress);
+    if (s->h.violated) return constraint_error(s);
+    return AVH_OK;
+}
+
+avh_status avh_compute_flux(avh_solver* s, double* host_out, double ws[3]) {
+    if (!s || !host_out || !ws) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    if ((r = pull(s))) return r;
+    StageParams P;
+    P.ctrl = s->ctrl;
+    P.ncells = s->ncells;
+    for (int d = 0; d < 3; ++d) P.n[d] = s->cfg.n[d];
+    P.bc = s->cfg.boundary;
+    P.gamma = s->cfg.gamma;
+    P.in = s->A;
+    P.u0 = s->A;
+    P.out = s->B;
+    P.mode = MODE_FLUX_ONLY;
+    AVH_HIP(s->stage(P, s->stream));
+    AVH_HIP(hipMemcpyAsync(host_out, s->B, (size_t)s->nv * s->ncells * sizeof(double), hipMemcpyDeviceToHost,
+                           s->stream));
+    if ((r = pull(s))) return r;
+    for (int d = 0; d < 3; ++d) {
+        double w;
+        std::memcpy(&w, &s->h.ws_bits[d], sizeof w);
+        ws[d] = w;
+        s->h.ws_bits[d] = 0;
+    }
+    return push(s);
+}
+
+}  // extern "C"

@@ -1,0 +1,453 @@
+// Fused FP64 finite-volume stage kernel.
+//
+// One launch = one RK stage over the whole grid.  Each workgroup owns a
+// TX x TY x TZ tile of interior cells:
+//   1. stage the tile plus a GHOST-wide halo of the stage input into LDS;
+//      halo cells outside the domain are read through the boundary map
+//      (periodic wrap / neumann mirror), which is the ghost fill of the
+//      reference done at load time; in the on-the-fly modes the stage input
+//      is formed while loading as U1 = L0*s + U0;
+//   2. per direction: one work item per interface, reconstructing the right
+//      face of the left cell and the left face of the right cell and
+//      evaluating the numerical flux; fluxes go to LDS, the interface wave
+//      speeds to a per-thread max;
+//   3. each thread accumulates the net flux of its cells in registers,
+//      out = 0 + ((-F[i+1/2]) + F[i-1/2]) per direction, exactly as
+//      NumericalFluxCPU::computeNetFlux does on a zeroed output;
+//   4. the RK stage combination of the integrator, the constraint check of
+//      the final stage, and the write of the result.
+// The CFL wave speed is reduced with wavefront shuffles and one atomicMax per
+// direction per workgroup (non-negative doubles order like their bit
+// patterns); the last workgroup to finish derives dt (or advances time) for
+// the next launch, so a whole time step runs without a host round trip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "avh_physics.hpp"
+
+#pragma clang fp contract(off)
+
+namespace avh {
+
+constexpr int MAX_ADJ = 4;
+
+// Device-resident step bookkeeping.  Written only by the last workgroup of a
+// launch (after every other workgroup has finished reading it) or by the host
+// between launches.
+struct Control {
+    unsigned long long ws_bits[3];  // running max wave speed per direction (double bits)
+    double t;                        // current time (TimestepInformation)
+    double dt;                       // dt of the step in flight
+    double scale;                    // dt / dx (RungeKutta*: output *= dt/cellLengths.x)
+    double last_dt;
+    double t_end;
+    double cfl;
+    double cell[3];
+    double adj_interval[MAX_ADJ];
+    double adj_start[MAX_ADJ];
+    long long adj_saved[MAX_ADJ];  // FixedIntervalWriter::numberSaved
+    long long steps;
+    int n_adj;
+    int active;         // 1 while the step in flight must run
+    int violated;       // sticky constraint failure
+    int saves_pending;  // fixed-interval save times reached, not yet acknowledged
+    int pause_on_save;
+    int block_violation;
+    unsigned int blocks_done;
+    int pad;
+};
+
+enum StageMode {
+    MODE_FLUX_ONLY = 0,  // out = net flux, wave speeds accumulated, nothing else
+    MODE_L = 1,          // first stage: out = L(U0), last block computes dt
+    MODE_RK2_2 = 2,      // in = U1;      out = ((L*s + U1) + U0) * 0.5, ends step
+    MODE_RK3_2 = 3,      // in = L0*s+U0; out = ((L*s + U1) * (1/3) + U0) * (3/4)
+    MODE_RK3_3 = 4       // in = U2;      out = ((L*s + U2) * 2 + U0) * (1/3), ends step
+};
+
+enum { BC_PERIODIC = 0, BC_NEUMANN = 1 };
+
+struct StageParams {
+    const double* in;  // stage input U_s, or L0 in the on-the-fly modes
+    const double* u0;  // U0 (on-the-fly input and RK combination)
+    double* out;       // stage output (may equal u0 in MODE_RK2_2 / MODE_RK3_3:
+                       // u0 is then read only at the cell being written)
+    Control* ctrl;
+    long long ncells;  // interior cells; variable stride of every buffer
+    int n[3];
+    int bc;
+    int mode;
+    double gamma;
+};
+
+// Boundary map of a coordinate in [-G, n+G) onto the interior: periodic wrap
+// (alsfvm/include/alsfvm/boundary/Periodic.hpp:24-42) or neumann mirror
+// (alsfvm/include/alsfvm/boundary/Neumann.hpp:24-35).  Applied per axis this
+// reproduces the reference's x -> y -> z fill order, corners included.
+__device__ __forceinline__ int bc_map(int g, int n, int bc) {
+    if (g < 0) return bc == BC_PERIODIC ? g + n : -1 - g;
+    if (g >= n) return bc == BC_PERIODIC ? g - n : 2 * n - 1 - g;
+    return g;
+}
+
+__device__ __forceinline__ double ws_max(double acc, double v) {
+    // std::max(acc, v): NaN candidates leave acc unchanged
+    return (acc < v) ? v : acc;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = ws_max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ unsigned int tile_id_xcd(unsigned int b, unsigned int nb) {
+    // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
+    // contiguous run of tiles so tiles sharing halos share an L2.
+    const unsigned int nfull = nb - nb % 8u;
+    if (b >= nfull) return b;
+    return (b % 8u) * (nfull / 8u) + b / 8u;
+}
+
+// Last-workgroup bookkeeping for MODE_L: dt = cfl / sum_d(ws_d / dx_d), then
+// the FixedIntervalWriter clips (alsfvm/src/integrator/Integrator.cpp:20-66,
+// alsfvm/src/io/FixedIntervalWriter.cpp:45-53).
+__device__ __forceinline__ void finish_dt(Control* c) {
+    double total = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double ws = __longlong_as_double((long long)__hip_atomic_load(
+            &c->ws_bits[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (c->cell[d] == 0) continue;
+        total += ws / c->cell[d];
+    }
+    double dt = c->cfl / total;
+    for (int i = 0; i < c->n_adj; ++i) {
+        if (c->adj_saved[i] > 0) {
+            const double next = (double)c->adj_saved[i] * c->adj_interval[i] + c->adj_start[i];
+            const double cand = next - c->t;
+            dt = (cand < dt) ? cand : dt;
+        }
+    }
+    c->dt = dt;
+    c->scale = dt / c->cell[0];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        __hip_atomic_store(&c->ws_bits[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Last-workgroup bookkeeping at the end of a step: t += dt, constraint flag,
+// then the writers' save counters (alsfvm/src/simulator/Simulator.cpp:81-85,
+// 180-214; alsfvm/src/io/FixedIntervalWriter.cpp:33-43) and the loop test
+// `t >= endTime` (Simulator.cpp:77-79).
+__device__ __forceinline__ void finish_step(Control* c) {
+    const int bad =
+        __hip_atomic_load(&c->block_violation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    c->t = c->t + c->dt;
+    c->steps = c->steps + 1;
+    c->last_dt = c->dt;
+    if (bad) {
+        c->violated = 1;
+    } else {
+        for (int i = 0; i < c->n_adj; ++i) {
+            if (c->t >= (double)c->adj_saved[i] * c->adj_interval[i] + c->adj_start[i]) {
+                c->adj_saved[i] = c->adj_saved[i] + 1;
+                c->saves_pending = c->saves_pending + 1;
+            }
+        }
+    }
+    __hip_atomic_store(&c->block_violation, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool more = !(c->t >= c->t_end) && !c->violated && !(c->pause_on_save && c->saves_pending > 0);
+    c->active = more ? 1 : 0;
+}
+
+// Returns true in exactly one workgroup: the last to arrive.
+__device__ __forceinline__ bool arrive_last(Control* c, unsigned int nblocks, int* s_flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned int prev = atomicAdd(&c->blocks_done, 1u);
+        const bool last = prev == nblocks - 1;
+        if (last) {
+            __threadfence();
+            c->blocks_done = 0;
+        }
+        *s_flag = last ? 1 : 0;
+    }
+    __syncthreads();
+    return *s_flag != 0;
+}
+
+template <class EQ, int RECON, class FLUX, int DIM, int TX, int TY, int TZ, int NT>
+struct StageKernel {
+    static constexpr int NV = EQ::NV;
+    static constexpr int G = ReconTraits<RECON>::GHOST;
+    static constexpr int HX = G;
+    static constexpr int HY = DIM >= 2 ? G : 0;
+    static constexpr int HZ = DIM >= 3 ? G : 0;
+    static constexpr int LX = TX + 2 * HX;
+    static constexpr int LY = TY + 2 * HY;
+    static constexpr int LZ = TZ + 2 * HZ;
+    static constexpr int LT = LX * LY * LZ;  // tile cells incl. halo
+    static constexpr int NI_X = (TX + 1) * TY * TZ;
+    static constexpr int NI_Y = DIM >= 2 ? TX * (TY + 1) * TZ : 0;
+    static constexpr int NI_Z = DIM >= 3 ? TX * TY * (TZ + 1) : 0;
+    static constexpr int NI_MAX = NI_X > NI_Y ? (NI_X > NI_Z ? NI_X : NI_Z) : (NI_Y > NI_Z ? NI_Y : NI_Z);
+    static constexpr int TC = TX * TY * TZ;
+    static constexpr int CPT = (TC + NT - 1) / NT;  // cells per thread in phase 3
+    static constexpr int LDS_BYTES = (NV * LT + NV * NI_MAX) * 8;
+
+    __device__ __forceinline__ static int lidx(int cx, int cy, int cz) {
+        return ((cz + HZ) * LY + (cy + HY)) * LX + (cx + HX);
+    }
+
+    template <int DIR>
+    __device__ __forceinline__ static void flux_pass(const double* su, double* sf, int ox, int oy, int oz,
+                                                     const int* n, double gamma, double& ws) {
+        constexpr int NI = DIR == 0 ? NI_X : (DIR == 1 ? NI_Y : NI_Z);
+        constexpr int SD = DIR == 0 ? 1 : (DIR == 1 ? LX : LX * LY);
+        for (int it = threadIdx.x; it < NI; it += NT) {
+            int cx, cy, cz, k;
+            if (DIR == 0) {
+                k = it % (TX + 1);
+                const int r = it / (TX + 1);
+                cy = r % TY;
+                cz = r / TY;
+                cx = k - 1;
+            } else if (DIR == 1) {
+                cx = it % TX;
+                const int r = it / TX;
+                k = r % (TY + 1);
+                cz = r / (TY + 1);
+                cy = k - 1;
+            } else {
+                cx = it % TX;
+                cy = (it / TX) % TY;
+                k = it / (TX * TY);
+                cz = k - 1;
+            }
+            // interface between cell a = (cx,cy,cz) and the next cell along DIR
+            const int ga = (DIR == 0 ? ox + cx : (DIR == 1 ? oy + cy : oz + cz));
+            const bool need = (ga < n[DIR]) && (oy + (DIR == 1 ? 0 : cy) < n[1]) &&
+                              (ox + (DIR == 0 ? 0 : cx) < n[0]) && (oz + (DIR == 2 ? 0 : cz) < n[2]);
+            if (!need) continue;
+            const int ia = lidx(cx, cy, cz);
+            double qa[NV], qb[NV];
+            recon_face<RECON, NV, true>(su, LT, ia, SD, qa);        // right face of a
+            recon_face<RECON, NV, false>(su, LT, ia + SD, SD, qb);  // left face of b
+            const typename EQ::All aL = EQ::make_all(qa, gamma);
+            const typename EQ::All aR = EQ::make_all(qb, gamma);
+            double F[NV];
+            const double w = FLUX::template flux<DIR, EQ>(aL, aR, gamma, F);
+            ws = ws_max(ws, w);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sf[v * NI_MAX + it] = F[v];
+        }
+    }
+
+    template <int DIR>
+    __device__ __forceinline__ static void accumulate(const double* sf, double (&acc)[CPT][NV]) {
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int c = threadIdx.x + m * NT;
+            if (TC % NT != 0 && c >= TC) continue;
+            const int cx = c % TX;
+            const int cy = (c / TX) % TY;
+            const int cz = c / (TX * TY);
+            int ip;
+            if (DIR == 0) ip = (cz * TY + cy) * (TX + 1) + cx + 1;
+            else if (DIR == 1) ip = (cz * (TY + 1) + cy + 1) * TX + cx;
+            else ip = ((cz + 1) * TY + cy) * TX + cx;
+            constexpr int STEP = DIR == 0 ? 1 : (DIR == 1 ? TX : TX * TY);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double fp = sf[v * NI_MAX + ip];
+                const double fm = sf[v * NI_MAX + ip - STEP];
+                // temp = -F; out[i] += temp[i] + (-1)*temp[i-1]
+                acc[m][v] = acc[m][v] + ((-fp) + fm);
+            }
+        }
+    }
+
+    __device__ static void run(const StageParams& P) {
+        __shared__ double lds[NV * LT + NV * NI_MAX];
+        __shared__ double s_ws[3][NT / 64];
+        __shared__ int s_flag;
+        double* su = lds;
+        double* sf = lds + NV * LT;
+
+        Control* ctrl = P.ctrl;
+        if (!ctrl->active && P.mode != MODE_FLUX_ONLY) return;  // step past the end: no-op
+
+        const int n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
+        const int ntx = (n0 + TX - 1) / TX, nty = (n1 + TY - 1) / TY;
+        const unsigned int nb = gridDim.x;
+        const unsigned int tile = tile_id_xcd(blockIdx.x, nb);
+        const int tx = tile % ntx;
+        const int ty = (tile / ntx) % nty;
+        const int tz = tile / (ntx * nty);
+        const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
+        const long long nc = P.ncells;
+        const bool otf = (P.mode == MODE_RK3_2);
+        const double s = (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) ? 0.0 : ctrl->scale;
+
+        // 1. stage input tile + halo into LDS through the boundary map
+        for (int p = threadIdx.x; p < LT; p += NT) {
+            const int lx = p % LX;
+            const int ly = (p / LX) % LY;
+            const int lz = p / (LX * LY);
+            int gx = ox + lx - HX, gy = oy + ly - HY, gz = oz + lz - HZ;
+            gx = gx < n0 + HX ? gx : n0 + HX - 1;
+            gy = gy < n1 + HY ? gy : n1 + HY - 1;
+            gz = gz < n2 + HZ ? gz : n2 + HZ - 1;
+            gx = bc_map(gx, n0, P.bc);
+            gy = DIM >= 2 ? bc_map(gy, n1, P.bc) : 0;
+            gz = DIM >= 3 ? bc_map(gz, n2, P.bc) : 0;
+            const long long gi = ((long long)gz * n1 + gy) * n0 + gx;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                double val = P.in[v * nc + gi];
+                if (otf) val = val * s + P.u0[v * nc + gi];
+                su[v * LT + p] = val;
+            }
+        }
+        __syncthreads();
+
+        double acc[CPT][NV];
+#pragma unroll
+        for (int m = 0; m < CPT; ++m)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[m][v] = 0.0;
+        double ws[3] = {0.0, 0.0, 0.0};
+
+        // 2./3. per direction: interface fluxes into LDS, then accumulate
+        flux_pass<0>(su, sf, ox, oy, oz, P.n, P.gamma, ws[0]);
+        __syncthreads();
+        accumulate<0>(sf, acc);
+        if (DIM >= 2) {
+            __syncthreads();
+            flux_pass<DIM >= 2 ? 1 : 0>(su, sf, ox, oy, oz, P.n, P.gamma, ws[1]);
+            __syncthreads();
+            accumulate<DIM >= 2 ? 1 : 0>(sf, acc);
+        }
+        if (DIM >= 3) {
+            __syncthreads();
+            flux_pass<DIM >= 3 ? 2 : 0>(su, sf, ox, oy, oz, P.n, P.gamma, ws[2]);
+            __syncthreads();
+            accumulate<DIM >= 3 ? 2 : 0>(sf, acc);
+        }
+
+        // 4. RK combination and write-back
+        bool bad = false;
+        const bool final_stage = (P.mode == MODE_RK2_2 || P.mode == MODE_RK3_3);
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            const int c = threadIdx.x + m * NT;
+            if (TC % NT != 0 && c >= TC) continue;
+            const int cx = c % TX;
+            const int cy = (c / TX) % TY;
+            const int cz = c / (TX * TY);
+            const int gx = ox + cx, gy = oy + cy, gz = oz + cz;
+            if (gx >= n0 || gy >= n1 || gz >= n2) continue;
+            const long long gi = ((long long)gz * n1 + gy) * n0 + gx;
+            const int li = lidx(cx, cy, cz);
+            double r[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double l = acc[m][v];
+                if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
+                    r[v] = l;
+                } else {
+                    const double us = su[v * LT + li];
+                    const double u0 = P.u0[v * nc + gi];
+                    double o = l * s;
+                    o = o + us;
+                    if (P.mode == MODE_RK2_2) {
+                        o = o + u0;
+                        o = o * 0.5;
+                    } else if (P.mode == MODE_RK3_2) {
+                        o = o * (1. / 3.);
+                        o = o + u0;
+                        o = o * (3. / 4.);
+                    } else {
+                        o = o * 2;
+                        o = o + u0;
+                        o = o * (1. / 3.);
+                    }
+                    r[v] = o;
+                }
+            }
+            if (final_stage) bad = bad || !EQ::obeys(r, P.gamma);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) P.out[v * nc + gi] = r[v];
+        }
+
+        const unsigned int nblocks = gridDim.x;
+        if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
+            // wave speed: wavefront reduction, one atomic per direction per workgroup
+            const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double w = wave_max(ws[d]);
+                if (lane == 0) s_ws[d][wid] = w;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                // one thread issues the workgroup's atomics so that its own
+                // fence in arrive_last() orders them before the arrival count
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    double w = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NT / 64; ++i) w = ws_max(w, s_ws[d][i]);
+                    if (w > 0.0)
+                        atomicMax(&ctrl->ws_bits[d], (unsigned long long)__double_as_longlong(w));
+                }
+            }
+            if (P.mode == MODE_L) {
+                if (arrive_last(ctrl, nblocks, &s_flag) && threadIdx.x == 0) finish_dt(ctrl);
+            }
+        } else if (final_stage) {
+            const bool any_bad = __syncthreads_or(bad);
+            if (any_bad && threadIdx.x == 0) atomicOr(&ctrl->block_violation, 1);
+            if (arrive_last(ctrl, nblocks, &s_flag) && threadIdx.x == 0) finish_step(ctrl);
+        }
+    }
+};
+
+template <class K>
+__global__ __launch_bounds__(256) void stage_kernel(StageParams P) {
+    K::run(P);
+}
+
+// Host-side launcher type: one per (equation, reconstruction, flux, dim).
+typedef hipError_t (*StageLauncher)(const StageParams& P, hipStream_t stream);
+
+template <class EQ, int RECON, class FLUX, int DIM>
+struct StageConfig;
+// tile shapes per dimension (cells): 1D 256, 2D 32x16, 3D 8x8x4
+template <class EQ, int RECON, class FLUX>
+struct StageConfig<EQ, RECON, FLUX, 1> {
+    typedef StageKernel<EQ, RECON, FLUX, 1, 256, 1, 1, 256> K;
+};
+template <class EQ, int RECON, class FLUX>
+struct StageConfig<EQ, RECON, FLUX, 2> {
+    typedef StageKernel<EQ, RECON, FLUX, 2, 32, 16, 1, 256> K;
+};
+template <class EQ, int RECON, class FLUX>
+struct StageConfig<EQ, RECON, FLUX, 3> {
+    typedef StageKernel<EQ, RECON, FLUX, 3, 8, 8, 4, 256> K;
+};
+
+template <class EQ, int RECON, class FLUX, int DIM>
+hipError_t launch_stage(const StageParams& P, hipStream_t stream) {
+    typedef typename StageConfig<EQ, RECON, FLUX, DIM>::K K;
+    const unsigned int ntx = (P.n[0] + K::LX - 2 * K::HX - 1) / (K::LX - 2 * K::HX);
+    const unsigned int nty = (P.n[1] + K::LY - 2 * K::HY - 1) / (K::LY - 2 * K::HY);
+    const unsigned int ntz = (P.n[2] + K::LZ - 2 * K::HZ - 1) / (K::LZ - 2 * K::HZ);
+    const unsigned int nb = ntx * nty * ntz;
+    hipLaunchKernelGGL(stage_kernel<K>, dim3(nb), dim3(256), 0, stream, P);
+    return hipGetLastError();
+}
+
+}  // namespace avh

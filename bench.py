@@ -1,0 +1,168 @@
+"""Benchmark of the hip platform on one MI355X.
+
+Workload: 2D compressible Euler (gamma 1.4) Kelvin-Helmholtz on the unit
+square, 1024 x 1024 cells, periodic, FP64, HLL3 flux + WENO2 + RK3-TVD,
+CFL 0.4, seeded band perturbations.  One step = one full RK3 time step
+(3 fused stage launches + on-device CFL dt) over the 1024^2 grid; the
+metric is FP64 cell-updates/s = cells * steps / wall time, the state
+resident in HBM before the timed region.
+
+Prints one JSON line with the roofline of the fused stage kernel (HIP event
+timing on the solver's stream) and the CPU baseline (the oracle -- a
+reference-structure restatement of the reference CPU path -- timed on a
+bounded number of steps of the same workload on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+N = 1024
+NV = 4
+CFL = 0.4
+
+
+def algorithmic_bytes_per_cell_step(nv):
+    # per RK3 step: stage 1 reads U0, writes L0 (2 arrays); stage 2 reads L0,
+    # U0, writes U2 (3); stage 3 reads U2, U0, writes U3 (3): 8 arrays of nv
+    # doubles per cell
+    return 8 * nv * 8
+
+
+def stage_bytes_per_cell(nv):
+    return [2 * nv * 8, 3 * nv * 8, 3 * nv * 8]
+
+
+def cpu_baseline(seconds_budget=15.0, n=N):
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if threads <= 0:
+        threads = min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle import pyoracle
+    from alsvinn_amd.initial_data import kelvin_helmholtz_2d, kh_params
+
+    sim = pyoracle.Simulator("euler2", "weno2", "hll3", "rungekutta3", "periodic", (n, n, 1), cfl=CFL)
+    sim.set_interior(kelvin_helmholtz_2d(n, kh_params(2024)))
+    sim.step()  # first touch / thread start-up outside the timing
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        sim.step()
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds_budget or steps >= 1000:
+            break
+    return {
+        "value": n * n * steps / el,
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d RK3 steps of the same 2D KH %dx%d HLL3+WENO2 workload (reference-structure "
+                  "CPU restatement, OpenMP, -O3 -ffp-contract=off), %.1f s" % (steps, n, n, el),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--profile-steps", type=int, default=100)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import alsvinn_amd
+    from alsvinn_amd.initial_data import kelvin_helmholtz_2d, kh_params
+
+    print("bench: building KH %dx%d state" % (N, N), file=sys.stderr, flush=True)
+    sim = alsvinn_amd.Simulator("euler2", "weno2", "hll3", "rungekutta3", "periodic", (N, N, 1), cfl=CFL)
+    sim.set_interior(kelvin_helmholtz_2d(N, kh_params(2024)))
+
+    sim.enqueue_steps(args.warmup)
+    sim.synchronize()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    sim.enqueue_steps(args.steps)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    prog = sim.progress()
+    if prog.violated:
+        raise SystemExit("constraint violation during the timed steps")
+    cells = N * N
+    value = cells * args.steps / el
+    ms_per_step = el * 1e3 / args.steps
+
+    # per-launch timing of the fused stage kernel (HIP events on the solver's stream)
+    import ctypes
+    L = alsvinn_amd.load()
+    ms = (ctypes.c_double * 4)()
+    nk = ctypes.c_int()
+    alsvinn_amd._abi.check(L.avh_time_step_kernels(sim._h, args.profile_steps, ms, ctypes.byref(nk)))
+    stage_ms = [ms[i] for i in range(nk.value)]
+    sb = stage_bytes_per_cell(NV)
+    avg_ms = sum(stage_ms) / len(stage_ms)
+    avg_bytes = cells * sum(sb) / len(sb)
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "stage_kernel_kh2d_1024_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_seconds)
+
+    out = {
+        "metric": "FP64 cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "2D compressible Euler Kelvin-Helmholtz %dx%d, periodic, HLL3 + WENO2 + RK3-TVD, "
+                        "CFL %.2f, gamma 1.4, FP64" % (N, N, CFL),
+            "cells": cells,
+            "step": "one RK3 time step (3 fused stage launches, on-device CFL dt)",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "avh::stage_kernel (fused reconstruct + flux + RK stage)",
+            "algorithmic_bytes_per_launch": avg_bytes,
+            "stage_ms": stage_ms,
+        },
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

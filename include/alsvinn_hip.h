@@ -1,0 +1,175 @@
+/*
+ * alsvinn_hip.h -- C ABI of the MI355X finite-volume engine (platform "hip").
+ *
+ * One RK time step of a hyperbolic conservation law on a Cartesian grid:
+ * reconstruction (none / weno2 / eno2 / eno3) -> numerical flux (godunov for
+ * burgers; hll / hll3 for euler1/2/3) -> net-flux divergence -> RK stage
+ * combination (forwardeuler / rungekutta2 / rungekutta3), with the CFL time
+ * step from an on-device max-wave-speed reduction and periodic / neumann
+ * ghost cells resolved on the device.  All arithmetic is IEEE FP64 in the
+ * operation order of the reference CPU path.
+ *
+ * Plain C types only.  Every call returns an avh_status; avh_last_error()
+ * gives a thread-local message for the last failing call on this thread.
+ * A solver is driven by one host thread (it owns one HIP stream); distinct
+ * solvers may be driven concurrently.
+ *
+ * Reference interfaces each entry point stands in for (files of
+ * alsvinn/alsvinn v0.5.4):
+ *   avh_solver_create   -> Simulator::Simulator + NumericalFluxFactory /
+ *                          ReconstructionFactory / IntegratorFactory /
+ *                          BoundaryFactory dispatch on platform
+ *                          (alsfvm/src/simulator/Simulator.cpp:26-64,
+ *                           alsfvm/src/numflux/NumericalFluxFactory.cpp:43-200,
+ *                           alsfvm/src/reconstruction/ReconstructionFactory.cpp:41-355,
+ *                           alsfvm/src/integrator/IntegratorFactory.cpp:31-44,
+ *                           alsfvm/src/boundary/BoundaryFactory.cpp:48-75)
+ *   avh_set_state       -> Simulator::setSimulationState
+ *                          (alsfvm/src/simulator/Simulator.cpp:100-103)
+ *   avh_get_state       -> the ghosted conserved Volume the writers receive
+ *                          (alsfvm/src/simulator/Simulator.cpp:164-170)
+ *   avh_add_fixed_interval_adjuster
+ *                       -> FixedIntervalWriter as a TimestepAdjuster
+ *                          (alsfvm/src/io/FixedIntervalWriter.cpp:24-53,
+ *                           alsfvm/src/config/SimulatorSetup.cpp:222-228)
+ *   avh_step            -> Simulator::performStep = incrementSolution +
+ *                          checkConstraints + callWriters
+ *                          (alsfvm/src/simulator/Simulator.cpp:81-85,180-214)
+ *   avh_advance         -> the driver loop `while (!atEnd()) performStep()`
+ *                          (alsvinncli/src/main.cpp:210-232)
+ *   avh_compute_flux    -> NumericalFlux::computeFlux over the whole domain
+ *                          (alsfvm/include/alsfvm/numflux/NumericalFlux.hpp:44-52)
+ */
+#ifndef ALSVINN_HIP_H
+#define ALSVINN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum avh_status {
+    AVH_OK = 0,
+    AVH_EINVAL = 1,      /* bad argument / unsupported combination        */
+    AVH_EHIP = 2,        /* a HIP runtime call failed                     */
+    AVH_ECONSTRAINT = 3, /* state violates the equation's constraints     */
+    AVH_ENOMEM = 4       /* device allocation failed                      */
+} avh_status;
+
+/* equation names as in the reference XML <equation> */
+enum { AVH_EQ_BURGERS = 0, AVH_EQ_EULER1 = 1, AVH_EQ_EULER2 = 2, AVH_EQ_EULER3 = 3 };
+/* <reconstruction> */
+enum { AVH_RECON_NONE = 0, AVH_RECON_WENO2 = 1, AVH_RECON_ENO2 = 2, AVH_RECON_ENO3 = 3 };
+/* <flux> */
+enum { AVH_FLUX_GODUNOV = 0, AVH_FLUX_HLL = 1, AVH_FLUX_HLL3 = 2 };
+/* <integrator> */
+enum { AVH_INT_FORWARDEULER = 0, AVH_INT_RUNGEKUTTA2 = 1, AVH_INT_RUNGEKUTTA3 = 2 };
+/* <boundary> */
+enum { AVH_BC_PERIODIC = 0, AVH_BC_NEUMANN = 1 };
+
+typedef struct avh_config {
+    int equation;       /* AVH_EQ_*                                        */
+    int reconstruction; /* AVH_RECON_*                                     */
+    int flux;           /* AVH_FLUX_*                                      */
+    int integrator;     /* AVH_INT_*                                       */
+    int boundary;       /* AVH_BC_*                                        */
+    int n[3];           /* interior cells per axis; 1 on inactive axes      */
+    double lower[3];    /* <lowerCorner>                                   */
+    double upper[3];    /* <upperCorner>                                   */
+    double gamma;       /* euler gamma (reference default 1.4)             */
+    double cfl;         /* CFL number                                      */
+} avh_config;
+
+typedef struct avh_ctx avh_ctx;
+typedef struct avh_solver avh_solver;
+
+/* Step counters and flags read back from the device. */
+typedef struct avh_progress {
+    double time;          /* current simulation time                        */
+    double last_dt;       /* dt of the last completed step                  */
+    int64_t steps;        /* number of completed steps                      */
+    int32_t violated;     /* 1 if the last step broke the constraints       */
+    int32_t saves_pending;/* number of fixed-interval saves hit, not acked  */
+} avh_progress;
+
+const char* avh_last_error(void);
+
+avh_status avh_ctx_create(int device, avh_ctx** ctx);
+avh_status avh_ctx_destroy(avh_ctx* ctx);
+
+avh_status avh_solver_create(avh_ctx* ctx, const avh_config* cfg, avh_solver** solver);
+avh_status avh_solver_destroy(avh_solver* solver);
+
+/* Number of ghost cells per active axis (2 weno2/eno2, 3 eno3, 1 none). */
+avh_status avh_ghost_cells(const avh_solver* solver, int* ghosts);
+/* Conserved-variable count (1 burgers, 2+nsd euler). */
+avh_status avh_num_vars(const avh_solver* solver, int* nvars);
+
+/* Host SoA state, reference Volume layout: nvars consecutive arrays, each
+ * (n[0]+2g)*(n[1]+2g')*(n[2]+2g'') doubles, x fastest, g' = g on active
+ * axes and 0 on inactive ones.  Ghost entries are ignored by set and filled
+ * with the boundary condition by get. */
+avh_status avh_set_state(avh_solver* solver, const double* host_ghosted);
+avh_status avh_get_state(avh_solver* solver, double* host_ghosted);
+/* Same, but the state is already in device memory (interior-only SoA,
+ * nvars * n[0]*n[1]*n[2] doubles).  The copy is device-to-device. */
+avh_status avh_set_state_device(avh_solver* solver, const double* dev_interior);
+avh_status avh_get_state_device(avh_solver* solver, double* dev_interior);
+
+/* Time bookkeeping: current time and step count (reference
+ * TimestepInformation).  Resets the device counters. */
+avh_status avh_set_time(avh_solver* solver, double time, int64_t steps);
+
+/* Register a FixedIntervalWriter-equivalent timestep adjuster: next save
+ * at numberSaved*interval + start_time, numberSaved starts at 0 when
+ * write_initial != 0 and at 1 otherwise.  At most 4 per solver. */
+avh_status avh_add_fixed_interval_adjuster(avh_solver* solver, double interval,
+                                           double start_time, int write_initial);
+
+/* One full time step (all RK stages, CFL dt, adjusters, constraint check,
+ * save counters).  Blocks until done.  Returns AVH_ECONSTRAINT if the new
+ * state violates the constraints. */
+avh_status avh_step(avh_solver* solver, double* dt_out);
+
+/* Advance until time >= t_end (reference atEnd), at most max_steps steps,
+ * enqueueing chunk steps at a time between host synchronisations.  Steps
+ * past the end are no-ops on the device.  If pause_on_save != 0 it returns
+ * after the step that reached a fixed-interval save time (progress.
+ * saves_pending > 0) so the caller can read the state; call
+ * avh_ack_saves() before continuing. */
+avh_status avh_advance(avh_solver* solver, double t_end, int64_t max_steps,
+                       int chunk, int pause_on_save, avh_progress* progress);
+avh_status avh_ack_saves(avh_solver* solver);
+avh_status avh_get_progress(avh_solver* solver, avh_progress* progress);
+
+/* Set the end time used by the device-side loop test (time >= t_end makes
+ * further enqueued steps no-ops).  Synchronises. */
+avh_status avh_set_end_time(avh_solver* solver, double t_end);
+/* Enqueue `steps` full time steps on the solver's stream without any host
+ * synchronisation (for timing and for callers that poll progress). */
+avh_status avh_enqueue_steps(avh_solver* solver, int64_t steps);
+avh_status avh_synchronize(avh_solver* solver);
+
+/* Run `steps` full time steps with a HIP event pair around every kernel
+ * launch on the solver's stream (the hipEvent analogue of the reference's
+ * ALSVINN_TIME_CUDA_BLOCK timers, alsutils/include/alsutils/timer/CudaTimer.hpp:11-48).
+ * ms_per_launch[k] receives the average duration in milliseconds of the
+ * k-th kernel of a step (k < *kernels_per_step <= 4), in launch order. */
+avh_status avh_time_step_kernels(avh_solver* solver, int64_t steps, double* ms_per_launch,
+                                 int* kernels_per_step);
+
+/* HIP stream the solver launches on (a hipStream_t as void*). */
+void* avh_stream(avh_solver* solver);
+
+/* NumericalFlux::computeFlux over the whole domain on the current state:
+ * writes the unscaled net-flux divergence (interior-only SoA) to
+ * host_interior_out and the per-direction max wave speeds to ws[3]. */
+avh_status avh_compute_flux(avh_solver* solver, double* host_interior_out, double ws[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALSVINN_HIP_H */

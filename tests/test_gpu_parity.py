@@ -1,0 +1,180 @@
+"""HIP path vs the CPU oracle on identical seeded inputs.
+
+Bar (stated per test): relative L1 per conserved variable <= 1e-10 on smooth
+runs and <= 1e-8 past shocks; the kernels are built to be bit-equal to the
+oracle, which the *_bitwise tests assert where the run is short enough that
+the oracle itself is the reference-structure CPU path.
+"""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+import alsvinn_amd
+from tests.cases import alternating_euler3, kh2d, rel_l1, smooth_state
+
+pytestmark = pytest.mark.gpu
+
+SMOOTH_TOL = 1e-10
+SHOCK_TOL = 1e-8
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_pair(oracle, equation, recon, flux, integ, bc, n, cfl, lower=(0, 0, 0), upper=(1, 1, 1)):
+    n = tuple(n) + (1,) * (3 - len(n))
+    o = oracle.Simulator(equation, recon, flux, integ, bc, n, lower, upper, 1.4, cfl)
+    h = alsvinn_amd.Simulator(equation, recon, flux, integ, bc, n, lower, upper, 1.4, cfl)
+    return o, h
+
+
+def run_both(oracle, equation, recon, flux, integ, bc, n, cfl, q0, steps=None, t_end=None, saves=None):
+    o, h = make_pair(oracle, equation, recon, flux, integ, bc, n, cfl)
+    o.set_interior(q0)
+    h.set_interior(q0)
+    if saves:
+        o.add_fixed_interval(t_end / saves)
+        h.add_fixed_interval_writer(t_end / saves)
+    if t_end is not None:
+        o.run(t_end, max_steps=steps if steps else 1 << 62)
+        p = h.run(t_end, max_steps=steps if steps else 1 << 62)
+        assert p.steps == o.steps
+    else:
+        for _ in range(steps):
+            o.step()
+        for _ in range(steps):
+            h.perform_step()
+    return o, h
+
+
+def test_burgers_sine_config1_full_run():
+    """Burgers sine N=1024, WENO2 + Godunov + RK3, CFL 0.45, periodic, to T=0.5
+    past the shock (t=1/(2 pi)), landing on T with a one-save writer."""
+    from oracle import pyoracle
+    N = 1024
+    x = (np.arange(N) + 0.5) / N
+    q0 = np.sin(2 * np.pi * x).reshape(1, 1, 1, N)
+    _say("case burgers N=1024 T=0.5")
+    o, h = run_both(pyoracle, "burgers", "weno2", "godunov", "rungekutta3", "periodic", (N,), 0.45, q0,
+                    t_end=0.5, saves=1)
+    uo = o.get_interior()
+    uh = h.get_interior()
+    assert h.get_current_time() == o.time == 0.5
+    err = rel_l1(uh, uo)
+    _say("steps", o.steps, "rel L1", err, "max abs", np.abs(uh - uo).max())
+    assert np.all(err <= SHOCK_TOL)
+    np.testing.assert_array_equal(uh, uo)
+    # ghosted export equals the oracle's ghosted volume (periodic fill)
+    np.testing.assert_array_equal(h.get_state(), o.get_state())
+
+
+def test_kh2d_64_rk3_bitwise():
+    from oracle import pyoracle
+    n = 64
+    q0 = kh2d(n)
+    _say("case kh2d 64 x 40 steps")
+    o, h = run_both(pyoracle, "euler2", "weno2", "hll3", "rungekutta3", "periodic", (n, n), 0.4, q0, steps=40)
+    err = rel_l1(h.get_interior(), o.get_interior())
+    _say("rel L1", err)
+    assert np.all(err <= SHOCK_TOL)
+    np.testing.assert_array_equal(h.get_interior(), o.get_interior())
+    np.testing.assert_array_equal(h.get_state(), o.get_state())
+    assert h.progress().time == o.time
+
+
+def test_euler3_16_rk3_bitwise():
+    from oracle import pyoracle
+    n = (16, 16, 16)
+    q0 = smooth_state("euler3", n, seed=3, jump=True)
+    _say("case euler3 16^3 x 4 steps")
+    o, h = run_both(pyoracle, "euler3", "weno2", "hll3", "rungekutta3", "periodic", n, 0.3, q0, steps=4)
+    np.testing.assert_array_equal(h.get_interior(), o.get_interior())
+
+
+def test_compute_flux_alternating_states_bitwise():
+    """CUDANumericalFluxTest::CompareAgainstCPU shape: euler3, alternating
+    states, neumann, no reconstruction; net flux and wave speeds bit-equal."""
+    from oracle import pyoracle
+    for flux in ("hll", "hll3"):
+        n = (10, 10, 10)
+        q0 = alternating_euler3(n)
+        o, h = make_pair(pyoracle, "euler3", "none", flux, "forwardeuler", "neumann", n, 0.5)
+        o.set_interior(q0)
+        h.set_interior(q0)
+        U = o.get_state()
+        out0 = np.full_like(U, 44.0)
+        out_o, ws_o = pyoracle.compute_flux(o.cfg, U, out0)
+        out_h, ws_h = h.compute_flux()
+        np.testing.assert_array_equal(ws_h, ws_o)
+        np.testing.assert_array_equal(out_h, o.interior(out_o))
+
+
+GRIDS = {
+    "burgers": [(70,), (24, 20), (10, 9, 12)],
+    "euler1": [(70,)],
+    "euler2": [(70,), (40, 20)],
+    "euler3": [(40,), (20, 18), (10, 9, 12)],
+}
+FLUX_OF = {"burgers": ["godunov"], "euler1": ["hll", "hll3"], "euler2": ["hll", "hll3"], "euler3": ["hll", "hll3"]}
+
+
+def _matrix():
+    out = []
+    for eq, grids in GRIDS.items():
+        for n in grids:
+            for recon in ("none", "weno2", "eno2", "eno3"):
+                for flux in FLUX_OF[eq]:
+                    for integ in ("forwardeuler", "rungekutta2", "rungekutta3"):
+                        for bc in ("periodic", "neumann"):
+                            out.append((eq, n, recon, flux, integ, bc))
+    return out
+
+
+def test_variant_matrix_bitwise():
+    """Every equation x grid dimension x reconstruction x flux x integrator x
+    boundary, 3 steps on small grids with partial tiles."""
+    from oracle import pyoracle
+    failures = []
+    cases = _matrix()
+    for i, (eq, n, recon, flux, integ, bc) in enumerate(cases):
+        q0 = smooth_state(eq, n, seed=i, jump=(i % 2 == 0))
+        _say("case", i, eq, n, recon, flux, integ, bc)
+        cfl = 0.3 if recon != "none" else 0.45
+        o, h = run_both(pyoracle, eq, recon, flux, integ, bc, n, cfl, q0, steps=3)
+        a, b = h.get_state(), o.get_state()
+        if not np.array_equal(a, b):
+            failures.append((eq, n, recon, flux, integ, bc, float(np.abs(a - b).max()),
+                             rel_l1(a, b).max()))
+    for f in failures:
+        _say("MISMATCH", f)
+    assert not failures, "%d of %d variants differ" % (len(failures), len(cases))
+
+
+def test_constraint_violation_raises_like_reference():
+    """Negative pressure after a step: both paths raise at the same step."""
+    from oracle import pyoracle
+    n = (32, 32)
+    q0 = smooth_state("euler2", n, seed=5)
+    q0[3] = q0[3] - 0.999 * (q0[3] - 0.5 * (q0[1] ** 2 + q0[2] ** 2) / q0[0])  # p ~ 0
+    q0[1] += 3.0 * np.sign(np.sin(np.arange(n[0]) * 0.7))[None, :]
+    o, h = make_pair(pyoracle, "euler2", "weno2", "hll3", "rungekutta3", "periodic", n, 0.9)
+    o.set_interior(q0)
+    h.set_interior(q0)
+    so = sh = None
+    for k in range(50):
+        try:
+            o.step()
+        except pyoracle.OracleError:
+            so = k
+            break
+    for k in range(50):
+        try:
+            h.perform_step()
+        except alsvinn_amd.ConstraintError:
+            sh = k
+            break
+    assert so is not None and so == sh
