@@ -65,6 +65,36 @@ int ghosts_of(int recon) {
     return 0;
 }
 
+// End of a step, one thread: t += dt, constraint flag, then the writers'
+// save counters (alsfvm/src/simulator/Simulator.cpp:81-85, 180-214;
+// alsfvm/src/io/FixedIntervalWriter.cpp:33-43) and the loop test
+// `t >= endTime` (Simulator.cpp:77-79); clears the wave-speed maxima for the
+// next step.
+__global__ __launch_bounds__(64) void finish_step_kernel(Control* c) {
+    if (threadIdx.x != 0 || !c->active) return;
+    const double dt = step_dt(c);
+    c->dt = dt;
+    c->scale = dt / c->cell[0];
+    c->t = c->t + dt;
+    c->steps = c->steps + 1;
+    c->last_dt = dt;
+    if (c->block_violation) {
+        c->violated = 1;
+    } else {
+        for (int i = 0; i < c->n_adj; ++i) {
+            if (c->t >= (double)c->adj_saved[i] * c->adj_interval[i] + c->adj_start[i]) {
+                c->adj_saved[i] = c->adj_saved[i] + 1;
+                c->saves_pending = c->saves_pending + 1;
+            }
+        }
+    }
+    c->block_violation = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c->ws_bits[d] = 0ull;
+    const bool more = !(c->t >= c->t_end) && !c->violated && !(c->pause_on_save && c->saves_pending > 0);
+    c->active = more ? 1 : 0;
+}
+
 enum EwMode { EW_FE_FINISH = 0, EW_RK2_U1 = 1 };
 
 // out = L*s + U0 element-wise (Volume *= dt/dx; += U0).  EW_FE_FINISH writes
@@ -72,10 +102,9 @@ enum EwMode { EW_FE_FINISH = 0, EW_RK2_U1 = 1 };
 template <class EQ>
 __global__ __launch_bounds__(256) void ew_kernel(const double* L, const double* u0, double* out,
                                                  Control* ctrl, long long nc, int mode, double gamma) {
-    __shared__ int s_flag;
     if (!ctrl->active) return;
     constexpr int NV = EQ::NV;
-    const double s = ctrl->scale;
+    const double s = step_dt(ctrl) / ctrl->cell[0];
     bool bad = false;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
          i += (long long)gridDim.x * blockDim.x) {
@@ -91,9 +120,7 @@ __global__ __launch_bounds__(256) void ew_kernel(const double* L, const double* 
         for (int v = 0; v < NV; ++v) out[v * nc + i] = r[v];
     }
     if (mode == EW_FE_FINISH) {
-        const bool any_bad = __syncthreads_or(bad);
-        if (any_bad && threadIdx.x == 0) atomicOr(&ctrl->block_violation, 1);
-        if (arrive_last(ctrl, gridDim.x, &s_flag) && threadIdx.x == 0) finish_step(ctrl);
+        if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&ctrl->block_violation, 1);
     }
 }
 
@@ -181,14 +208,16 @@ static void recompute_active(Control& h) {
     h.active = (!(h.t >= h.t_end) && !h.violated && !(h.pause_on_save && h.saves_pending > 0)) ? 1 : 0;
 }
 
-// Optional per-launch event timing (avh_time_step_kernels).
+// Optional per-launch event timing (avh_time_step_kernels): an event after
+// every launch of every step, all recorded before one synchronisation.
 struct KernelTimer {
-    hipEvent_t ev[8];
-    int count = 0;
+    std::vector<hipEvent_t> ev;
+    size_t count = 0;
 };
 
 static avh_status mark(avh_solver* s, KernelTimer* t) {
     if (!t) return AVH_OK;
+    if (t->count >= t->ev.size()) return fail(AVH_EINVAL, "kernel timer overflow");
     AVH_HIP(hipEventRecord(t->ev[t->count++], s->stream));
     return AVH_OK;
 }
@@ -237,6 +266,9 @@ static avh_status enqueue_step(avh_solver* s, KernelTimer* tm = nullptr) {
         AVH_HIP(s->stage(P, s->stream));
         if ((r = mark(s, tm))) return r;
     }
+    hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s->stream, s->ctrl);
+    AVH_HIP(hipGetLastError());
+    if ((r = mark(s, tm))) return r;
     return AVH_OK;
 }
 
@@ -479,28 +511,33 @@ avh_status avh_synchronize(avh_solver* s) {
 void* avh_stream(avh_solver* s) { return s ? (void*)s->stream : nullptr; }
 
 avh_status avh_time_step_kernels(avh_solver* s, int64_t steps, double* ms, int* kernels) {
-    if (!s || !ms || !kernels || steps < 1) return fail(AVH_EINVAL, "bad argument");
+    if (!s || !ms || !kernels || steps < 1 || steps > 100000) return fail(AVH_EINVAL, "bad argument");
     avh_status r = use_device(s);
     if (r) return r;
-    const int nk = s->cfg.integrator == AVH_INT_FORWARDEULER ? 2 : 3;
+    // launches per step: the stages (plus the element-wise pass of forward
+    // euler / rk2) and the end-of-step kernel
+    const int nk = s->cfg.integrator == AVH_INT_FORWARDEULER ? 3 : 4;
     KernelTimer t;
-    for (int i = 0; i <= nk; ++i) AVH_HIP(hipEventCreate(&t.ev[i]));
-    double acc[4] = {0, 0, 0, 0};
-    for (int64_t k = 0; k < steps && r == AVH_OK; ++k) {
-        t.count = 0;
-        r = enqueue_step(s, &t);
-        if (r) break;
-        if (hipEventSynchronize(t.ev[nk]) != hipSuccess) {
-            r = fail(AVH_EHIP, "hipEventSynchronize failed");
-            break;
-        }
-        for (int i = 0; i < nk; ++i) {
-            float f = 0;
-            hipEventElapsedTime(&f, t.ev[i], t.ev[i + 1]);
-            acc[i] += f;
-        }
+    t.ev.resize((size_t)steps * (nk + 1));
+    size_t made = 0;
+    for (; made < t.ev.size(); ++made)
+        if (hipEventCreate(&t.ev[made]) != hipSuccess) break;
+    if (made < t.ev.size()) {
+        for (size_t i = 0; i < made; ++i) hipEventDestroy(t.ev[i]);
+        return fail(AVH_EHIP, "hipEventCreate failed");
     }
-    for (int i = 0; i <= nk; ++i) hipEventDestroy(t.ev[i]);
+    for (int64_t k = 0; k < steps && r == AVH_OK; ++k) r = enqueue_step(s, &t);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!r && hipEventSynchronize(t.ev[t.count - 1]) != hipSuccess) r = fail(AVH_EHIP, "hipEventSynchronize failed");
+    if (!r) {
+        for (int64_t k = 0; k < steps; ++k)
+            for (int i = 0; i < nk; ++i) {
+                float f = 0;
+                hipEventElapsedTime(&f, t.ev[k * (nk + 1) + i], t.ev[k * (nk + 1) + i + 1]);
+                acc[i] += f;
+            }
+    }
+    for (auto& e : t.ev) hipEventDestroy(e);
     if (r) return r;
     for (int i = 0; i < nk; ++i) ms[i] = acc[i] / (double)steps;
     *kernels = nk;

@@ -18,8 +18,10 @@
 //      the final stage, and the write of the result.
 // The CFL wave speed is reduced with wavefront shuffles and one atomicMax per
 // direction per workgroup (non-negative doubles order like their bit
-// patterns); the last workgroup to finish derives dt (or advances time) for
-// the next launch, so a whole time step runs without a host round trip.
+// patterns); the launches after the first stage derive dt from those maxima
+// themselves, and a one-wave kernel closes the step (time, save counters,
+// loop test), so a whole time step runs without a host round trip and
+// without grid-wide fences.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -31,9 +33,8 @@ namespace avh {
 
 constexpr int MAX_ADJ = 4;
 
-// Device-resident step bookkeeping.  Written only by the last workgroup of a
-// launch (after every other workgroup has finished reading it) or by the host
-// between launches.
+// Device-resident step bookkeeping.  Written by the host between launches,
+// by atomics (wave-speed maxima, violation flag) and by finish_step_kernel.
 struct Control {
     unsigned long long ws_bits[3];  // running max wave speed per direction (double bits)
     double t;                        // current time (TimestepInformation)
@@ -53,8 +54,6 @@ struct Control {
     int saves_pending;  // fixed-interval save times reached, not yet acknowledged
     int pause_on_save;
     int block_violation;
-    unsigned int blocks_done;
-    int pad;
 };
 
 enum StageMode {
@@ -109,15 +108,17 @@ __device__ __forceinline__ unsigned int tile_id_xcd(unsigned int b, unsigned int
     return (b % 8u) * (nfull / 8u) + b / 8u;
 }
 
-// Last-workgroup bookkeeping for MODE_L: dt = cfl / sum_d(ws_d / dx_d), then
-// the FixedIntervalWriter clips (alsfvm/src/integrator/Integrator.cpp:20-66,
-// alsfvm/src/io/FixedIntervalWriter.cpp:45-53).
-__device__ __forceinline__ void finish_dt(Control* c) {
+// dt of the step in flight: cfl / sum_d(ws_d / dx_d), then the
+// FixedIntervalWriter clips (alsfvm/src/integrator/Integrator.cpp:20-66,
+// alsfvm/src/io/FixedIntervalWriter.cpp:45-53).  ws_d are the maxima the
+// first stage accumulated; every launch after it evaluates this same
+// expression, so all of them use the identical dt without a grid-wide
+// barrier or fence.
+__device__ __forceinline__ double step_dt(const Control* c) {
     double total = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double ws = __longlong_as_double((long long)__hip_atomic_load(
-            &c->ws_bits[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const double ws = __longlong_as_double((long long)c->ws_bits[d]);
         if (c->cell[d] == 0) continue;
         total += ws / c->cell[d];
     }
@@ -129,53 +130,7 @@ __device__ __forceinline__ void finish_dt(Control* c) {
             dt = (cand < dt) ? cand : dt;
         }
     }
-    c->dt = dt;
-    c->scale = dt / c->cell[0];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-        __hip_atomic_store(&c->ws_bits[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Last-workgroup bookkeeping at the end of a step: t += dt, constraint flag,
-// then the writers' save counters (alsfvm/src/simulator/Simulator.cpp:81-85,
-// 180-214; alsfvm/src/io/FixedIntervalWriter.cpp:33-43) and the loop test
-// `t >= endTime` (Simulator.cpp:77-79).
-__device__ __forceinline__ void finish_step(Control* c) {
-    const int bad =
-        __hip_atomic_load(&c->block_violation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    c->t = c->t + c->dt;
-    c->steps = c->steps + 1;
-    c->last_dt = c->dt;
-    if (bad) {
-        c->violated = 1;
-    } else {
-        for (int i = 0; i < c->n_adj; ++i) {
-            if (c->t >= (double)c->adj_saved[i] * c->adj_interval[i] + c->adj_start[i]) {
-                c->adj_saved[i] = c->adj_saved[i] + 1;
-                c->saves_pending = c->saves_pending + 1;
-            }
-        }
-    }
-    __hip_atomic_store(&c->block_violation, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const bool more = !(c->t >= c->t_end) && !c->violated && !(c->pause_on_save && c->saves_pending > 0);
-    c->active = more ? 1 : 0;
-}
-
-// Returns true in exactly one workgroup: the last to arrive.
-__device__ __forceinline__ bool arrive_last(Control* c, unsigned int nblocks, int* s_flag) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned int prev = atomicAdd(&c->blocks_done, 1u);
-        const bool last = prev == nblocks - 1;
-        if (last) {
-            __threadfence();
-            c->blocks_done = 0;
-        }
-        *s_flag = last ? 1 : 0;
-    }
-    __syncthreads();
-    return *s_flag != 0;
+    return dt;
 }
 
 template <class EQ, int RECON, class FLUX, int DIM, int TX, int TY, int TZ, int NT>
@@ -272,7 +227,6 @@ struct StageKernel {
     __device__ static void run(const StageParams& P) {
         __shared__ double lds[NV * LT + NV * NI_MAX];
         __shared__ double s_ws[3][NT / 64];
-        __shared__ int s_flag;
         double* su = lds;
         double* sf = lds + NV * LT;
 
@@ -289,7 +243,7 @@ struct StageKernel {
         const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
         const long long nc = P.ncells;
         const bool otf = (P.mode == MODE_RK3_2);
-        const double s = (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) ? 0.0 : ctrl->scale;
+        const double s = (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) ? 0.0 : step_dt(ctrl) / ctrl->cell[0];
 
         // 1. stage input tile + halo into LDS through the boundary map
         for (int p = threadIdx.x; p < LT; p += NT) {
@@ -382,9 +336,9 @@ struct StageKernel {
             for (int v = 0; v < NV; ++v) P.out[v * nc + gi] = r[v];
         }
 
-        const unsigned int nblocks = gridDim.x;
         if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
-            // wave speed: wavefront reduction, one atomic per direction per workgroup
+            // wave speed: wavefront reduction, then one atomicMax per direction
+            // per workgroup (non-negative doubles order like their bit patterns)
             const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -392,25 +346,15 @@ struct StageKernel {
                 if (lane == 0) s_ws[d][wid] = w;
             }
             __syncthreads();
-            if (threadIdx.x == 0) {
-                // one thread issues the workgroup's atomics so that its own
-                // fence in arrive_last() orders them before the arrival count
+            if (threadIdx.x < 3) {
+                double w = 0.0;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    double w = 0.0;
-#pragma unroll
-                    for (int i = 0; i < NT / 64; ++i) w = ws_max(w, s_ws[d][i]);
-                    if (w > 0.0)
-                        atomicMax(&ctrl->ws_bits[d], (unsigned long long)__double_as_longlong(w));
-                }
-            }
-            if (P.mode == MODE_L) {
-                if (arrive_last(ctrl, nblocks, &s_flag) && threadIdx.x == 0) finish_dt(ctrl);
+                for (int i = 0; i < NT / 64; ++i) w = ws_max(w, s_ws[threadIdx.x][i]);
+                if (w > 0.0)
+                    atomicMax(&ctrl->ws_bits[threadIdx.x], (unsigned long long)__double_as_longlong(w));
             }
         } else if (final_stage) {
-            const bool any_bad = __syncthreads_or(bad);
-            if (any_bad && threadIdx.x == 0) atomicOr(&ctrl->block_violation, 1);
-            if (arrive_last(ctrl, nblocks, &s_flag) && threadIdx.x == 0) finish_step(ctrl);
+            if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&ctrl->block_violation, 1);
         }
     }
 };

@@ -72,7 +72,22 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run only a plain device copy of MIB MiB (torch copy kernel), to calibrate "
+                         "the FETCH_SIZE / WRITE_SIZE counters against a known byte count")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                          "dispatches": 3}), flush=True)
+        return
 
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -108,7 +123,9 @@ def main():
     ms = (ctypes.c_double * 4)()
     nk = ctypes.c_int()
     alsvinn_amd._abi.check(L.avh_time_step_kernels(sim._h, args.profile_steps, ms, ctypes.byref(nk)))
-    stage_ms = [ms[i] for i in range(nk.value)]
+    launch_ms = [ms[i] for i in range(nk.value)]
+    stage_ms = launch_ms[:3]          # the three fused stage launches
+    finish_ms = launch_ms[3]          # one-wave end-of-step bookkeeping kernel
     sb = stage_bytes_per_cell(NV)
     avg_ms = sum(stage_ms) / len(stage_ms)
     avg_bytes = cells * sum(sb) / len(sb)
@@ -157,6 +174,7 @@ def main():
             "kernel": "avh::stage_kernel (fused reconstruct + flux + RK stage)",
             "algorithmic_bytes_per_launch": avg_bytes,
             "stage_ms": stage_ms,
+            "end_of_step_kernel_ms": finish_ms,
         },
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,

@@ -156,7 +156,8 @@ avh_status avh_synchronize(avh_solver* solver);
  * launch on the solver's stream (the hipEvent analogue of the reference's
  * ALSVINN_TIME_CUDA_BLOCK timers, alsutils/include/alsutils/timer/CudaTimer.hpp:11-48).
  * ms_per_launch[k] receives the average duration in milliseconds of the
- * k-th kernel of a step (k < *kernels_per_step <= 4), in launch order. */
+ * k-th kernel of a step (k < *kernels_per_step <= 4), in launch order; the
+ * last one is the end-of-step bookkeeping kernel. */
 avh_status avh_time_step_kernels(avh_solver* solver, int64_t steps, double* ms_per_launch,
                                  int* kernels_per_step);
 

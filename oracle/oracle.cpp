@@ -660,6 +660,40 @@ struct FixedInterval {
 };
 
 /* ------------------------------------------------------------------ */
+/* {ForwardEuler,RungeKutta2,RungeKutta3}::performSubstep
+ * (alsfvm/src/integrator/ForwardEuler.cpp:32-51, RungeKutta2.cpp:37-67,
+ *  RungeKutta3.cpp:49-86): system, dt at substep 0, then output *= dt/dx.x,
+ * += U_substep and the stage combination, as whole-volume vector ops. */
+template <class Sys, class Dt>
+static real integrator_substep(int integrator, std::vector<Volume>& vols, int s, real dt, real cellx,
+                               Sys&& system, Dt&& compute_dt) {
+    Volume& out = vols[s + 1];
+    real ws[3] = {0, 0, 0};
+    system(vols[s], ws, out);
+    if (integrator == AVH_INT_FORWARDEULER || s == 0) dt = compute_dt(ws);
+    const real scale = dt / cellx;
+    vol_mul(out, scale);
+    vol_add(out, vols[s]);
+    if (integrator == AVH_INT_RUNGEKUTTA2) {
+        if (s == 1) {
+            vol_add(out, vols[0]);
+            vol_mul(out, 0.5);
+        }
+    } else if (integrator == AVH_INT_RUNGEKUTTA3) {
+        if (s == 1) {
+            vol_mul(out, 1. / 3.);
+            vol_add(out, vols[0]);
+            vol_mul(out, 3. / 4.);
+        } else if (s == 2) {
+            vol_mul(out, 2);
+            vol_add(out, vols[0]);
+            vol_mul(out, 1. / 3.);
+        }
+    }
+    return dt;
+}
+
+/* ------------------------------------------------------------------ */
 /* Simulator with Integrator (FE / RK2 / RK3) */
 
 struct Simulator {
@@ -689,32 +723,10 @@ struct Simulator {
         return dt;
     }
 
-    /* {ForwardEuler,RungeKutta2,RungeKutta3}::performSubstep */
     real substep(int s, real dt) {
-        Volume& out = vols[s + 1];
-        real ws[3] = {0, 0, 0};
-        system(vols[s], ws, out);
-        if (cfg.integrator == AVH_INT_FORWARDEULER || s == 0) dt = compute_timestep(ws);
-        const real scale = dt / cell[0];
-        vol_mul(out, scale);
-        vol_add(out, vols[s]);
-        if (cfg.integrator == AVH_INT_RUNGEKUTTA2) {
-            if (s == 1) {
-                vol_add(out, vols[0]);
-                vol_mul(out, 0.5);
-            }
-        } else if (cfg.integrator == AVH_INT_RUNGEKUTTA3) {
-            if (s == 1) {
-                vol_mul(out, 1. / 3.);
-                vol_add(out, vols[0]);
-                vol_mul(out, 3. / 4.);
-            } else if (s == 2) {
-                vol_mul(out, 2);
-                vol_add(out, vols[0]);
-                vol_mul(out, 1. / 3.);
-            }
-        }
-        return dt;
+        return integrator_substep(
+            cfg.integrator, vols, s, dt, cell[0], [this](const Volume& U, real* ws, Volume& out) { system(U, ws, out); },
+            [this](const real* ws) { return compute_timestep(ws); });
     }
 
     bool check_constraints() const {
@@ -1085,6 +1097,98 @@ int oracle_from_primitive(int equation, double gamma, const double* prim, double
         for (int k = 0; k < nv; ++k) cons[(size_t)k * ncells + i] = q[k];
     }
     return 0;
+}
+
+/* Point flux of an AllVariables given field by field: a = {rho, m[nsd],
+ * E, p, u[nsd]} (EulerEquationTest builds such states directly,
+ * test/library_tests/src/fvm/equation/EulerEquationTest.cpp:27). */
+int oracle_point_flux_all(int nsd, int dir, const double* a, double* F) {
+#define ORACLE_PA(NSD)                                                               \
+    if (nsd == NSD) {                                                                \
+        Euler<NSD> eq;                                                               \
+        Euler<NSD>::All v;                                                           \
+        v.rho = a[0];                                                                \
+        for (int c = 0; c < NSD; ++c) v.m[c] = a[1 + c];                             \
+        v.E = a[1 + NSD];                                                            \
+        v.p = a[2 + NSD];                                                            \
+        for (int c = 0; c < NSD; ++c) v.u[c] = a[3 + NSD + c];                       \
+        if (dir == 0) eq.point_flux<0>(v, F);                                        \
+        else if (dir == 1) eq.point_flux<NSD >= 2 ? 1 : 0>(v, F);                    \
+        else eq.point_flux<NSD >= 3 ? 2 : 0>(v, F);                                  \
+        return 0;                                                                    \
+    }
+    ORACLE_PA(1)
+    ORACLE_PA(2)
+    ORACLE_PA(3)
+#undef ORACLE_PA
+    return AVH_EINVAL;
+}
+
+/* Numerical flux between two AllVariables given field by field (HLLTest
+ * passes AllVariables(1, {2,3,4}, 5, 6, {7,8,9}) straight to the flux,
+ * test/library_tests/src/fvm/flux/HLLTest.cpp:24-71). */
+double oracle_flux_all(int nsd, int flux, int dir, double gamma, const double* aL, const double* aR, double* F) {
+#define ORACLE_FA(NSD)                                                                 \
+    if (nsd == NSD) {                                                                  \
+        Euler<NSD> eq;                                                                 \
+        eq.gamma = gamma;                                                              \
+        Euler<NSD>::All L, R;                                                          \
+        const double* src[2] = {aL, aR};                                               \
+        Euler<NSD>::All* dst[2] = {&L, &R};                                            \
+        for (int k = 0; k < 2; ++k) {                                                  \
+            const double* a = src[k];                                                  \
+            dst[k]->rho = a[0];                                                        \
+            for (int c = 0; c < NSD; ++c) dst[k]->m[c] = a[1 + c];                     \
+            dst[k]->E = a[1 + NSD];                                                    \
+            dst[k]->p = a[2 + NSD];                                                    \
+            for (int c = 0; c < NSD; ++c) dst[k]->u[c] = a[3 + NSD + c];               \
+        }                                                                              \
+        if (flux == AVH_FLUX_HLL3) {                                                   \
+            if (dir == 0) return HLL3::flux<0>(eq, L, R, F);                           \
+            if (dir == 1) return HLL3::flux<NSD >= 2 ? 1 : 0>(eq, L, R, F);            \
+            return HLL3::flux<NSD >= 3 ? 2 : 0>(eq, L, R, F);                          \
+        }                                                                              \
+        if (dir == 0) return HLL::flux<0>(eq, L, R, F);                                \
+        if (dir == 1) return HLL::flux<NSD >= 2 ? 1 : 0>(eq, L, R, F);                 \
+        return HLL::flux<NSD >= 3 ? 2 : 0>(eq, L, R, F);                               \
+    }
+    ORACLE_FA(1)
+    ORACLE_FA(2)
+    ORACLE_FA(3)
+#undef ORACLE_FA
+    return -1;
+}
+
+/* IntegratorConvergenceTest's ODESystem (test/library_tests/src/fvm/
+ * integrator/IntegratorConvergenceTest.cpp:28-43): output = h*u, wave
+ * speeds (1,0,0), cell sizes (1,1,1), cfl 1, so each substep integrates
+ * du/dt = u with step h through the integrator's own vector arithmetic.
+ * Runs `steps` steps from u0 and returns u. */
+double oracle_integrate_ode(int integrator, double h, long steps, double u0) {
+    const int substeps = integrator == AVH_INT_FORWARDEULER ? 1 : (integrator == AVH_INT_RUNGEKUTTA2 ? 2 : 3);
+    std::vector<Volume> vols;
+    for (int i = 0; i <= substeps; ++i) {
+        vols.emplace_back(1, 1, 1, 1, 0, 0, 0);
+        vols.back().d[0] = u0;
+    }
+    auto sys = [h](const Volume& U, real* ws, Volume& out) {
+        out.d[0] = h * U.d[0];
+        ws[0] = 1;
+        ws[1] = 0;
+        ws[2] = 0;
+    };
+    auto cdt = [](const real* ws) {
+        real total = 0;
+        const real cell[3] = {1, 1, 1};
+        for (int d = 0; d < 3; ++d) total += ws[d] / cell[d];
+        return real(1) / total;
+    };
+    for (long k = 0; k < steps; ++k) {
+        real dt = 1;
+        for (int s = 0; s < substeps; ++s) dt = integrator_substep(integrator, vols, s, dt, 1.0, sys, cdt);
+        std::swap(vols[0], vols[substeps]);
+    }
+    return vols[0].d[0];
 }
 
 /* std::mt19937_64 draws (alsuq/src/generator/STLMersenne.cpp:38-85 uses the

@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ORACLE_LIB selects another build of the oracle (the mutation tests load
+# deliberately broken copies through it).
+_LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 EQ = {"burgers": 0, "euler1": 1, "euler2": 2, "euler3": 3}
 RECON = {"none": 0, "weno2": 1, "eno2": 2, "eno3": 3}
@@ -104,6 +106,11 @@ def lib():
         L.oracle_apply_boundary.argtypes = [P(OracleConfig), ctypes.c_int, P(d)]
         L.oracle_obeys.argtypes = [ctypes.c_void_p, P(d)]
         L.oracle_from_primitive.argtypes = [ctypes.c_int, d, P(d), P(d), ctypes.c_size_t]
+        L.oracle_point_flux_all.argtypes = [ctypes.c_int, ctypes.c_int, P(d), P(d)]
+        L.oracle_flux_all.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, d, P(d), P(d), P(d)]
+        L.oracle_flux_all.restype = d
+        L.oracle_integrate_ode.argtypes = [ctypes.c_int, d, ctypes.c_long, d]
+        L.oracle_integrate_ode.restype = d
         L.oracle_mt19937_64.argtypes = [ctypes.c_uint64] * 3 + [P(ctypes.c_uint64)]
         L.oracle_uniform01.argtypes = [ctypes.c_uint64] * 3 + [P(d)]
         _lib = L
@@ -304,3 +311,23 @@ def uniform01(seed, skip, n):
     out = np.empty(n)
     lib().oracle_uniform01(seed, skip, n, _dp(out))
     return out
+
+
+def point_flux_all(nsd, direction, all_vars):
+    """all_vars = (rho, m..., E, p, u...)"""
+    a = np.ascontiguousarray(all_vars, dtype=np.float64)
+    F = np.zeros(nsd + 2)
+    lib().oracle_point_flux_all(int(nsd), int(direction), _dp(a), _dp(F))
+    return F
+
+
+def flux_all(nsd, flux, direction, left_all, right_all, gamma=1.4):
+    aL = np.ascontiguousarray(left_all, dtype=np.float64)
+    aR = np.ascontiguousarray(right_all, dtype=np.float64)
+    F = np.zeros(nsd + 2)
+    ws = lib().oracle_flux_all(int(nsd), FLUX[flux], int(direction), float(gamma), _dp(aL), _dp(aR), _dp(F))
+    return F, ws
+
+
+def integrate_ode(integrator, h, steps, u0=1.0):
+    return lib().oracle_integrate_ode(INTEGRATOR[integrator], float(h), int(steps), float(u0))
