@@ -199,13 +199,21 @@ struct HLL3Flux {
         uc[NSD + 1] = ek;
 #pragma unroll
         for (int i = 0; i < NV; ++i) fk[i] = left ? fl[i] : fr[i];
+        // subsonic interfaces (sL < 0 < sR) take the star branch; when the
+        // whole wavefront is subsonic the other three branches are skipped
+        const bool star = !(sl == 0) && !(sl > 0) && !(sr < 0);
+        if (__all(star)) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            double v = fk[i] + sk * (mid[i] - uc[i]);
-            v = (sr < 0) ? fr[i] : v;
-            v = (sl > 0) ? fl[i] : v;
-            v = (sl == 0) ? 0.0 : v;
-            F[i] = v;
+            for (int i = 0; i < NV; ++i) F[i] = fk[i] + sk * (mid[i] - uc[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double v = fk[i] + sk * (mid[i] - uc[i]);
+                v = (sr < 0) ? fr[i] : v;
+                v = (sl > 0) ? fl[i] : v;
+                v = (sl == 0) ? 0.0 : v;
+                F[i] = v;
+            }
         }
         return fmax(fabs(sl), fabs(sr));
     }

@@ -25,6 +25,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "avh_physics.hpp"
 
 #pragma clang fp contract(off)
@@ -149,6 +151,7 @@ struct StageKernel {
     static constexpr int NI_Z = DIM >= 3 ? TX * TY * (TZ + 1) : 0;
     static constexpr int NI_MAX = NI_X > NI_Y ? (NI_X > NI_Z ? NI_X : NI_Z) : (NI_Y > NI_Z ? NI_Y : NI_Z);
     static constexpr int TC = TX * TY * TZ;
+    static constexpr int TXC = TX, TYC = TY, TZC = TZ;
     static constexpr int CPT = (TC + NT - 1) / NT;  // cells per thread in phase 3
     static constexpr int LDS_BYTES = (NV * LT + NV * NI_MAX) * 8;
 
@@ -362,36 +365,6 @@ struct StageKernel {
 template <class K>
 __global__ __launch_bounds__(256) void stage_kernel(StageParams P) {
     K::run(P);
-}
-
-// Host-side launcher type: one per (equation, reconstruction, flux, dim).
-typedef hipError_t (*StageLauncher)(const StageParams& P, hipStream_t stream);
-
-template <class EQ, int RECON, class FLUX, int DIM>
-struct StageConfig;
-// tile shapes per dimension (cells): 1D 256, 2D 32x16, 3D 8x8x4
-template <class EQ, int RECON, class FLUX>
-struct StageConfig<EQ, RECON, FLUX, 1> {
-    typedef StageKernel<EQ, RECON, FLUX, 1, 256, 1, 1, 256> K;
-};
-template <class EQ, int RECON, class FLUX>
-struct StageConfig<EQ, RECON, FLUX, 2> {
-    typedef StageKernel<EQ, RECON, FLUX, 2, 32, 16, 1, 256> K;
-};
-template <class EQ, int RECON, class FLUX>
-struct StageConfig<EQ, RECON, FLUX, 3> {
-    typedef StageKernel<EQ, RECON, FLUX, 3, 8, 8, 4, 256> K;
-};
-
-template <class EQ, int RECON, class FLUX, int DIM>
-hipError_t launch_stage(const StageParams& P, hipStream_t stream) {
-    typedef typename StageConfig<EQ, RECON, FLUX, DIM>::K K;
-    const unsigned int ntx = (P.n[0] + K::LX - 2 * K::HX - 1) / (K::LX - 2 * K::HX);
-    const unsigned int nty = (P.n[1] + K::LY - 2 * K::HY - 1) / (K::LY - 2 * K::HY);
-    const unsigned int ntz = (P.n[2] + K::LZ - 2 * K::HZ - 1) / (K::LZ - 2 * K::HZ);
-    const unsigned int nb = ntx * nty * ntz;
-    hipLaunchKernelGGL(stage_kernel<K>, dim3(nb), dim3(256), 0, stream, P);
-    return hipGetLastError();
 }
 
 }  // namespace avh

@@ -1,0 +1,350 @@
+// Fused FP64 stage kernel for 3D grids: a TX x TY column tile marches along z.
+//
+// Per z-plane k of its chunk [z0, z1) a workgroup
+//   * writes plane k (its own columns from registers, the x/y halo ring from
+//     HBM through the boundary map) into an LDS plane buffer,
+//   * evaluates the x- and y-interface fluxes of the plane from LDS (one work
+//     item per interface, as in the 2D kernel) and accumulates them per cell,
+//   * evaluates the z-interface flux F(k+1/2) of each column in registers from
+//     a sliding window of planes [k+1-HW, k+1+HW] and the right face of
+//     plane k kept from the previous iteration; F(k-1/2) is likewise kept,
+//   * adds ((-F_z(k+1/2)) + F_z(k-1/2)) after the x and y terms -- the same
+//     x, y, z accumulation order as NumericalFluxCPU::computeFlux
+//     (alsfvm/src/numflux/NumericalFluxCPU.cpp:170-221) -- and applies the RK
+//     stage combination.
+// Each plane is read once per workgroup plus its x/y halo ring; no z halo is
+// ever reloaded, and the tile has no z-direction quantisation loss.
+#pragma once
+#include "avh_stage.hpp"
+
+#pragma clang fp contract(off)
+
+namespace avh {
+
+// ENO / WENO2 / none on a register window: u[i][v], stencil centre `c`
+// (compile-time), unit stride.  Selects replace ENO's data-dependent index.
+template <int RECON, int NV, bool RIGHT, int WZ>
+__device__ __forceinline__ void recon_face_reg(const double (&u)[WZ][NV], const int c, double* out) {
+    if (RECON == RECON_NONE) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out[v] = u[c][v];
+    } else if (RECON == RECON_WENO2) {
+        const double eps = 1e-8;
+        const double w0 = u[c][0];
+        const double dp = u[c + 1][0] - w0;
+        const double b0 = dp * dp;
+        const double dm = w0 - u[c - 1][0];
+        const double b1 = dm * dm;
+        const double d0 = 3 * (eps + b0) * (eps + b0);
+        const double d1 = 3 * (eps + b1) * (eps + b1);
+        if (RIGHT) {
+            const double a0 = 2 / d0;
+            const double a1 = 1 / d1;
+            const double w0r = a0 / (a0 + a1);
+            const double w1r = a1 / (a0 + a1);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                out[v] = 0.5 * (w0r * u[c + 1][v] + (3 * w1r + w0r) * u[c][v] - w1r * u[c - 1][v]);
+        } else {
+            const double a0 = 1 / d0;
+            const double a1 = 2 / d1;
+            const double w0l = a0 / (a0 + a1);
+            const double w1l = a1 / (a0 + a1);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                out[v] = 0.5 * (w1l * u[c - 1][v] + (3 * w0l + w1l) * u[c][v] - w0l * u[c + 1][v]);
+        }
+    } else {
+        constexpr int ORDER = RECON == RECON_ENO2 ? 2 : 3;
+        int shift = 0;
+        {
+            const double ddr = u[c + 1][0] - u[c][0];
+            const double ddl = u[c][0] - u[c - 1][0];
+            if (fabs(ddl) < fabs(ddr)) shift++;
+        }
+        if (ORDER == 3) {
+            const double d0p = u[c + 2 < WZ ? c + 2 : WZ - 1][0] - u[c + 1][0];
+            const double d00 = u[c + 1][0] - u[c][0];
+            const double d0m = u[c][0] - u[c - 1][0];
+            const double ddr = d0p - d00;
+            const double ddl = d00 - d0m;
+            if (fabs(ddl) < fabs(ddr)) shift++;
+        }
+        const int r = RIGHT ? shift + 1 : shift;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < ORDER; ++j) {
+                // value at offset j - shift, shift in [0, ORDER-1]
+                double val = u[c + j][v];
+#pragma unroll
+                for (int sft = 1; sft < ORDER; ++sft) {
+                    const int idx = c + j - sft;
+                    val = (shift == sft) ? u[idx < 0 ? 0 : idx][v] : val;
+                }
+                acc += eno_coef<ORDER>(r, j) * val;
+            }
+            out[v] = acc;
+        }
+    }
+}
+
+template <class EQ, int RECON, class FLUX, int TX, int TY, int LZ>
+struct MarchKernel3 {
+    static constexpr int NV = EQ::NV;
+    static constexpr int NT = TX * TY;  // one column per thread
+    static constexpr int G = ReconTraits<RECON>::GHOST;
+    static constexpr int HW = G - 1 > 1 ? G - 1 : 1;  // z-window half width
+    static constexpr int WZ = 2 * HW + 1;              // planes k+1-HW .. k+1+HW
+    static constexpr int LXP = TX + 2 * G;
+    static constexpr int LYP = TY + 2 * G;
+    static constexpr int LP = LXP * LYP;  // plane cells incl. halo
+    static constexpr int NIX = (TX + 1) * TY;
+    static constexpr int NIY = TX * (TY + 1);
+    static constexpr int NIMAX = NIX > NIY ? NIX : NIY;
+    static constexpr int NHALO = LP - TX * TY;
+
+    __device__ __forceinline__ static long long gidx(int x, int y, int z, int n0, int n1) {
+        return ((long long)z * n1 + y) * n0 + x;
+    }
+
+    // one input value at a (boundary-mapped) global cell
+    __device__ __forceinline__ static void fetch(const StageParams& P, bool otf, double s, long long gi, double* q) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double val = P.in[v * P.ncells + gi];
+            if (otf) val = val * s + P.u0[v * P.ncells + gi];
+            q[v] = val;
+        }
+    }
+
+    template <int DIR>
+    __device__ __forceinline__ static void plane_pass(const double* sp, double* sf, int ox, int oy, const int* n,
+                                                      double gamma, double& ws) {
+        constexpr int NI = DIR == 0 ? NIX : NIY;
+        constexpr int SD = DIR == 0 ? 1 : LXP;
+        for (int it = threadIdx.x; it < NI; it += NT) {
+            int cx, cy;
+            if (DIR == 0) {
+                cx = it % (TX + 1) - 1;
+                cy = it / (TX + 1);
+            } else {
+                cx = it % TX;
+                cy = it / TX - 1;
+            }
+            const bool need = (ox + cx < n[0]) && (oy + cy < n[1]);
+            if (!need) continue;
+            const int ia = (cy + G) * LXP + (cx + G);
+            double qa[NV], qb[NV];
+            recon_face<RECON, NV, true>(sp, LP, ia, SD, qa);
+            recon_face<RECON, NV, false>(sp, LP, ia + SD, SD, qb);
+            const typename EQ::All aL = EQ::make_all(qa, gamma);
+            const typename EQ::All aR = EQ::make_all(qb, gamma);
+            double F[NV];
+            const double w = FLUX::template flux<DIR, EQ>(aL, aR, gamma, F);
+            ws = ws_max(ws, w);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sf[v * NIMAX + it] = F[v];
+        }
+    }
+
+    __device__ static void run(const StageParams& P) {
+        __shared__ double sp[NV * LP];
+        __shared__ double sf[NV * NIMAX];
+        __shared__ double s_ws[3][NT / 64];
+
+        Control* ctrl = P.ctrl;
+        if (!ctrl->active && P.mode != MODE_FLUX_ONLY) return;
+
+        const int n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
+        const int ntx = (n0 + TX - 1) / TX, nty = (n1 + TY - 1) / TY;
+        const unsigned int tile = tile_id_xcd(blockIdx.x, gridDim.x);
+        const int tx = tile % ntx;
+        const int ty = (tile / ntx) % nty;
+        const int tz = tile / (ntx * nty);
+        const int ox = tx * TX, oy = ty * TY, z0 = tz * LZ;
+        const int z1 = (z0 + LZ < n2) ? z0 + LZ : n2;
+        const bool otf = (P.mode == MODE_RK3_2);
+        const double s = (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) ? 0.0 : step_dt(ctrl) / ctrl->cell[0];
+
+        // this thread's column.  A column past the grid edge (partial tile)
+        // sits where the tile's x/y halo lies, so it loads the boundary image
+        // of that position, exactly as the halo ring does.
+        const int cx = threadIdx.x % TX, cy = threadIdx.x / TX;
+        const bool col_ok = (ox + cx < n0) && (oy + cy < n1);
+        const int gxc = bc_map(ox + cx < n0 + G ? ox + cx : n0 + G - 1, n0, P.bc);
+        const int gyc = bc_map(oy + cy < n1 + G ? oy + cy : n1 + G - 1, n1, P.bc);
+
+        double ws[3] = {0.0, 0.0, 0.0};
+        double w[WZ][NV];
+        double rprev[NV], fprev[NV];
+
+        // prologue: window centred at z0-1 -> R(z0-1); then centred at z0 ->
+        // L(z0), R(z0), F_z(z0-1/2); then centred at z0+1 for the loop
+#pragma unroll
+        for (int i = 0; i < WZ; ++i) {
+            const int gz = bc_map(z0 - 1 - HW + i, n2, P.bc);
+            fetch(P, otf, s, gidx(gxc, gyc, gz, n0, n1), w[i]);
+        }
+        double rm1[NV];
+        recon_face_reg<RECON, NV, true, WZ>(w, HW, rm1);
+#pragma unroll
+        for (int i = 0; i < WZ - 1; ++i)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
+        fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
+        {
+            double l0[NV];
+            recon_face_reg<RECON, NV, false, WZ>(w, HW, l0);
+            recon_face_reg<RECON, NV, true, WZ>(w, HW, rprev);
+            const typename EQ::All aL = EQ::make_all(rm1, P.gamma);
+            const typename EQ::All aR = EQ::make_all(l0, P.gamma);
+            const double wz = FLUX::template flux<2, EQ>(aL, aR, P.gamma, fprev);
+            if (col_ok) ws[2] = ws_max(ws[2], wz);
+        }
+#pragma unroll
+        for (int i = 0; i < WZ - 1; ++i)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
+        fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + 1 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
+
+        bool bad = false;
+        const bool final_stage = (P.mode == MODE_RK2_2 || P.mode == MODE_RK3_3);
+        for (int k = z0; k < z1; ++k) {
+            // plane k into LDS: own column from the window (w[HW-1] is plane k),
+            // the halo ring through the boundary map
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sp[v * LP + (cy + G) * LXP + (cx + G)] = w[HW - 1][v];
+            for (int h = threadIdx.x; h < NHALO; h += NT) {
+                // enumerate the ring: rows above and below the tile, then the side columns
+                int lx, ly;
+                if (h < G * LXP) {
+                    ly = h / LXP;
+                    lx = h % LXP;
+                } else if (h < 2 * G * LXP) {
+                    ly = TY + G + (h - G * LXP) / LXP;
+                    lx = (h - G * LXP) % LXP;
+                } else {
+                    const int r = h - 2 * G * LXP;  // 2G side cells per interior row
+                    ly = G + r / (2 * G);
+                    const int c2 = r % (2 * G);
+                    lx = c2 < G ? c2 : TX + c2;
+                }
+                int gx = ox + lx - G, gy = oy + ly - G;
+                gx = gx < n0 + G ? gx : n0 + G - 1;
+                gy = gy < n1 + G ? gy : n1 + G - 1;
+                gx = bc_map(gx, n0, P.bc);
+                gy = bc_map(gy, n1, P.bc);
+                double q[NV];
+                fetch(P, otf, s, gidx(gx, gy, k, n0, n1), q);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sp[v * LP + ly * LXP + lx] = q[v];
+            }
+            __syncthreads();
+
+            double acc[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+            plane_pass<0>(sp, sf, ox, oy, P.n, P.gamma, ws[0]);
+            __syncthreads();
+            {
+                const int ip = cy * (TX + 1) + cx + 1;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] = acc[v] + ((-sf[v * NIMAX + ip]) + sf[v * NIMAX + ip - 1]);
+            }
+            __syncthreads();
+            plane_pass<1>(sp, sf, ox, oy, P.n, P.gamma, ws[1]);
+            __syncthreads();
+            {
+                const int ip = (cy + 1) * TX + cx;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] = acc[v] + ((-sf[v * NIMAX + ip]) + sf[v * NIMAX + ip - TX]);
+            }
+            // z: F(k+1/2) from R(k) (kept) and L(k+1) (window centred at k+1)
+            double fz[NV], rnext[NV];
+            {
+                double lz[NV];
+                recon_face_reg<RECON, NV, false, WZ>(w, HW, lz);
+                recon_face_reg<RECON, NV, true, WZ>(w, HW, rnext);
+                const typename EQ::All aL = EQ::make_all(rprev, P.gamma);
+                const typename EQ::All aR = EQ::make_all(lz, P.gamma);
+                const double wz = FLUX::template flux<2, EQ>(aL, aR, P.gamma, fz);
+                if (col_ok) ws[2] = ws_max(ws[2], wz);
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] = acc[v] + ((-fz[v]) + fprev[v]);
+
+            if (col_ok) {
+                const long long gi = gidx(ox + cx, oy + cy, k, n0, n1);
+                double r[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const double l = acc[v];
+                    if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
+                        r[v] = l;
+                    } else {
+                        const double us = w[HW - 1][v];
+                        const double u0 = P.u0[v * P.ncells + gi];
+                        double o = l * s;
+                        o = o + us;
+                        if (P.mode == MODE_RK2_2) {
+                            o = o + u0;
+                            o = o * 0.5;
+                        } else if (P.mode == MODE_RK3_2) {
+                            o = o * (1. / 3.);
+                            o = o + u0;
+                            o = o * (3. / 4.);
+                        } else {
+                            o = o * 2;
+                            o = o + u0;
+                            o = o * (1. / 3.);
+                        }
+                        r[v] = o;
+                    }
+                }
+                if (final_stage) bad = bad || !EQ::obeys(r, P.gamma);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) P.out[v * P.ncells + gi] = r[v];
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                fprev[v] = fz[v];
+                rprev[v] = rnext[v];
+            }
+            // advance the window to centre k+2
+#pragma unroll
+            for (int i = 0; i < WZ - 1; ++i)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
+            if (k + 1 < z1) fetch(P, otf, s, gidx(gxc, gyc, bc_map(k + 2 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
+            __syncthreads();
+        }
+
+        if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
+            const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double wv = wave_max(ws[d]);
+                if (lane == 0) s_ws[d][wid] = wv;
+            }
+            __syncthreads();
+            if (threadIdx.x < 3) {
+                double wv = 0.0;
+#pragma unroll
+                for (int i = 0; i < NT / 64; ++i) wv = ws_max(wv, s_ws[threadIdx.x][i]);
+                if (wv > 0.0)
+                    atomicMax(&ctrl->ws_bits[threadIdx.x], (unsigned long long)__double_as_longlong(wv));
+            }
+        } else if (final_stage) {
+            if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&ctrl->block_violation, 1);
+        }
+    }
+};
+
+template <class K>
+__global__ __launch_bounds__(256) void march_kernel3(StageParams P) {
+    K::run(P);
+}
+
+}  // namespace avh

@@ -2,8 +2,29 @@
 // reconstruction, flux and grid dimension the equation supports.
 #pragma once
 #include "avh_stage.hpp"
+#include "avh_stage3d.hpp"
 
 namespace avh {
+
+// Host-side launcher type: one per (equation, reconstruction, flux, dim).
+typedef hipError_t (*StageLauncher)(const StageParams& P, hipStream_t stream);
+
+// 1D: 256-cell tiles; 2D: 32x16 tiles; 3D: 32x8 column tiles marching over
+// 32 planes.
+template <class EQ, int RECON, class FLUX, int DIM>
+hipError_t launch_stage(const StageParams& P, hipStream_t stream) {
+    if constexpr (DIM == 3) {
+        typedef MarchKernel3<EQ, RECON, FLUX, 32, 8, 32> K;
+        const unsigned int nb = ((P.n[0] + 31) / 32) * ((P.n[1] + 7) / 8) * ((P.n[2] + 31) / 32);
+        hipLaunchKernelGGL(march_kernel3<K>, dim3(nb), dim3(K::NT), 0, stream, P);
+    } else {
+        typedef typename std::conditional<DIM == 1, StageKernel<EQ, RECON, FLUX, 1, 256, 1, 1, 256>,
+                                          StageKernel<EQ, RECON, FLUX, 2, 32, 16, 1, 256>>::type K;
+        const unsigned int nb = ((P.n[0] + K::TXC - 1) / K::TXC) * ((P.n[1] + K::TYC - 1) / K::TYC);
+        hipLaunchKernelGGL(stage_kernel<K>, dim3(nb), dim3(256), 0, stream, P);
+    }
+    return hipGetLastError();
+}
 
 template <class EQ, class FLUX, int DIM>
 StageLauncher pick_recon(int recon) {

@@ -68,10 +68,12 @@ def kelvin_helmholtz_3d(n, params, gamma=1.4):
     each of x and z (80 numbers: a0x, a1x, b0x, b1x, a0z, a1z, b0z, b1z)."""
     params = np.asarray(params, dtype=np.float64)
     x = midpoints(n)
-    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
     p4 = params.reshape(8, 10)
-    d0 = kh_perturbation(X, p4[0], p4[2]) + kh_perturbation(Z, p4[4], p4[6])
-    d1 = kh_perturbation(X, p4[1], p4[3]) + kh_perturbation(Z, p4[5], p4[7])
+    # the interfaces depend on (z, x) only: evaluate on the plane, broadcast over y
+    Zp, Xp = np.meshgrid(x, x, indexing="ij")
+    d0 = (kh_perturbation(Xp, p4[0], p4[2]) + kh_perturbation(Zp, p4[4], p4[6]))[:, None, :]
+    d1 = (kh_perturbation(Xp, p4[1], p4[3]) + kh_perturbation(Zp, p4[5], p4[7]))[:, None, :]
+    Y = x[None, :, None]
     band = (Y > 0.25 + d0) & (Y < 0.75 + d1)
     rho = np.where(band, 2.0, 1.0)
     ux = np.where(band, -0.5, 0.5)

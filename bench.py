@@ -1,11 +1,11 @@
 """Benchmark of the hip platform on one MI355X.
 
-Workload: 2D compressible Euler (gamma 1.4) Kelvin-Helmholtz on the unit
-square, 1024 x 1024 cells, periodic, FP64, HLL3 flux + WENO2 + RK3-TVD,
-CFL 0.4, seeded band perturbations.  One step = one full RK3 time step
-(3 fused stage launches + on-device CFL dt) over the 1024^2 grid; the
-metric is FP64 cell-updates/s = cells * steps / wall time, the state
-resident in HBM before the timed region.
+Default workload: 2D compressible Euler (gamma 1.4) Kelvin-Helmholtz on the
+unit square, 1024 x 1024 cells, periodic, FP64, HLL3 flux + WENO2 + RK3-TVD,
+CFL 0.4, seeded band perturbations (--workload kh3d: the 3D 256^3 case).
+One step = one full RK3 time step (3 fused stage launches + on-device CFL
+dt) over the grid; the metric is FP64 cell-updates/s = cells * steps / wall
+time, the state resident in HBM before the timed region.
 
 Prints one JSON line with the roofline of the fused stage kernel (HIP event
 timing on the solver's stream) and the CPU baseline (the oracle -- a
@@ -19,9 +19,17 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0
-N = 1024
-NV = 4
-CFL = 0.4
+
+WORKLOADS = {
+    # the bench line
+    "kh2d": dict(equation="euler2", n=(1024, 1024, 1), nv=4, cfl=0.4,
+                 desc="2D compressible Euler Kelvin-Helmholtz 1024x1024, periodic, HLL3 + WENO2 + RK3-TVD, "
+                      "CFL 0.40, gamma 1.4, FP64"),
+    # 3D KH, working set beyond the 256 MB MALL
+    "kh3d": dict(equation="euler3", n=(256, 256, 256), nv=5, cfl=0.3,
+                 desc="3D compressible Euler Kelvin-Helmholtz 256^3, periodic, HLL3 + WENO2 + RK3-TVD, "
+                      "CFL 0.30, gamma 1.4, FP64"),
+}
 
 
 def algorithmic_bytes_per_cell_step(nv):
@@ -35,16 +43,22 @@ def stage_bytes_per_cell(nv):
     return [2 * nv * 8, 3 * nv * 8, 3 * nv * 8]
 
 
-def cpu_baseline(seconds_budget=15.0, n=N):
+def initial_state(wl):
+    from alsvinn_amd.initial_data import kelvin_helmholtz_2d, kelvin_helmholtz_3d, kh_params
+    if wl["equation"] == "euler2":
+        return kelvin_helmholtz_2d(wl["n"][0], kh_params(2024))
+    return kelvin_helmholtz_3d(wl["n"][0], kh_params(2024, count=80))
+
+
+def cpu_baseline(wl, seconds_budget=15.0):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     if threads <= 0:
         threads = min(16, os.cpu_count() or 1)
         os.environ["OMP_NUM_THREADS"] = str(threads)
     from oracle import pyoracle
-    from alsvinn_amd.initial_data import kelvin_helmholtz_2d, kh_params
 
-    sim = pyoracle.Simulator("euler2", "weno2", "hll3", "rungekutta3", "periodic", (n, n, 1), cfl=CFL)
-    sim.set_interior(kelvin_helmholtz_2d(n, kh_params(2024)))
+    sim = pyoracle.Simulator(wl["equation"], "weno2", "hll3", "rungekutta3", "periodic", wl["n"], cfl=wl["cfl"])
+    sim.set_interior(initial_state(wl))
     sim.step()  # first touch / thread start-up outside the timing
     steps = 0
     t0 = time.perf_counter()
@@ -54,13 +68,14 @@ def cpu_baseline(seconds_budget=15.0, n=N):
         el = time.perf_counter() - t0
         if el >= seconds_budget or steps >= 1000:
             break
+    cells = wl["n"][0] * wl["n"][1] * wl["n"][2]
     return {
-        "value": n * n * steps / el,
+        "value": cells * steps / el,
         "unit": "cell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": "%d RK3 steps of the same 2D KH %dx%d HLL3+WENO2 workload (reference-structure "
-                  "CPU restatement, OpenMP, -O3 -ffp-contract=off), %.1f s" % (steps, n, n, el),
+        "sample": "%d RK3 steps of the same workload (%s) on the reference-structure CPU restatement "
+                  "(OpenMP, -O3 -ffp-contract=off), %.1f s" % (steps, "x".join(map(str, wl["n"])), el),
     }
 
 
@@ -72,6 +87,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="kh2d")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (torch copy kernel), to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters against a known byte count")
@@ -94,13 +110,13 @@ def main():
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import alsvinn_amd
-    from alsvinn_amd.initial_data import kelvin_helmholtz_2d, kh_params
 
-    print("bench: building KH %dx%d state" % (N, N), file=sys.stderr, flush=True)
-    sim = alsvinn_amd.Simulator("euler2", "weno2", "hll3", "rungekutta3", "periodic", (N, N, 1), cfl=CFL)
-    sim.set_interior(kelvin_helmholtz_2d(N, kh_params(2024)))
+    wl = WORKLOADS[args.workload]
+    NV = wl["nv"]
+    print("bench: building %s state" % args.workload, file=sys.stderr, flush=True)
+    sim = alsvinn_amd.Simulator(wl["equation"], "weno2", "hll3", "rungekutta3", "periodic", wl["n"], cfl=wl["cfl"])
+    sim.set_interior(initial_state(wl))
 
     sim.enqueue_steps(args.warmup)
     sim.synchronize()
@@ -113,7 +129,7 @@ def main():
     prog = sim.progress()
     if prog.violated:
         raise SystemExit("constraint violation during the timed steps")
-    cells = N * N
+    cells = wl["n"][0] * wl["n"][1] * wl["n"][2]
     value = cells * args.steps / el
     ms_per_step = el * 1e3 / args.steps
 
@@ -132,7 +148,8 @@ def main():
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
 
     traffic = None
-    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "stage_kernel_kh2d_1024_traffic.json")
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "stage_kernel_%s_traffic.json" % args.workload)
     if os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get("bytes_per_launch")
@@ -142,7 +159,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_seconds)
+        cpu = cpu_baseline(wl, args.cpu_seconds)
 
     out = {
         "metric": "FP64 cell-updates/s",
@@ -158,8 +175,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "2D compressible Euler Kelvin-Helmholtz %dx%d, periodic, HLL3 + WENO2 + RK3-TVD, "
-                        "CFL %.2f, gamma 1.4, FP64" % (N, N, CFL),
+            "workload": wl["desc"],
             "cells": cells,
             "step": "one RK3 time step (3 fused stage launches, on-device CFL dt)",
             "parallelism": "single GPU",
