@@ -35,6 +35,16 @@ SYMBOLS = [
     "avh_time_step_kernels",
     "avh_stream",
     "avh_compute_flux",
+    "avh_batch_create",
+    "avh_batch_destroy",
+    "avh_batch_add_fixed_interval_adjuster",
+    "avh_batch_set_state",
+    "avh_batch_get_state",
+    "avh_batch_set_kh2d",
+    "avh_batch_advance",
+    "avh_batch_get_progress",
+    "avh_batch_get_statistics",
+    "avh_uq_generate",
 ]
 
 AVH_OK, AVH_EINVAL, AVH_EHIP, AVH_ECONSTRAINT, AVH_ENOMEM = range(5)
@@ -63,6 +73,10 @@ class AvhProgress(ctypes.Structure):
         ("violated", ctypes.c_int32),
         ("saves_pending", ctypes.c_int32),
     ]
+
+
+class AvhGroup(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3), ("nsamples", ctypes.c_int)]
 
 
 class LibraryMissing(RuntimeError):
@@ -111,6 +125,16 @@ def load():
         "avh_synchronize": [vp],
         "avh_compute_flux": [vp, vp, P(d)],
         "avh_time_step_kernels": [vp, i64, P(d), P(ctypes.c_int)],
+        "avh_batch_create": [vp, P(AvhConfig), ctypes.c_int, P(AvhGroup), d, ctypes.c_int, ctypes.c_int, P(vp)],
+        "avh_batch_destroy": [vp],
+        "avh_batch_add_fixed_interval_adjuster": [vp, d, d, ctypes.c_int],
+        "avh_batch_set_state": [vp, ctypes.c_int, ctypes.c_int, vp],
+        "avh_batch_get_state": [vp, ctypes.c_int, ctypes.c_int, vp],
+        "avh_batch_set_kh2d": [vp, ctypes.c_int, vp],
+        "avh_batch_advance": [vp, i64, ctypes.c_int, P(i64), P(i64)],
+        "avh_batch_get_progress": [vp, ctypes.c_int, ctypes.c_int, P(AvhProgress)],
+        "avh_batch_get_statistics": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, P(i64)],
+        "avh_uq_generate": [ctypes.c_int, P(ctypes.c_int), P(d), P(d), i64, i64, vp],
     }
     for name, args in protos.items():
         f = getattr(L, name)

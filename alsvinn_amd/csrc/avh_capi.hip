@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/alsvinn_hip.h"
+#include "avh_internal.hpp"
 #include "avh_stage_inst.hpp"
 
 #pragma clang fp contract(off)
@@ -30,40 +31,6 @@
 using namespace avh;
 
 namespace {
-
-thread_local std::string g_last_error;
-
-avh_status fail(avh_status st, const std::string& msg) {
-    g_last_error = msg;
-    return st;
-}
-
-#define AVH_HIP(call)                                                                      \
-    do {                                                                                   \
-        hipError_t e_ = (call);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail(AVH_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
-    } while (0)
-
-int nvars_of(int eq) {
-    switch (eq) {
-    case AVH_EQ_BURGERS: return 1;
-    case AVH_EQ_EULER1: return 3;
-    case AVH_EQ_EULER2: return 4;
-    case AVH_EQ_EULER3: return 5;
-    }
-    return 0;
-}
-
-int ghosts_of(int recon) {
-    switch (recon) {
-    case AVH_RECON_NONE: return 1;
-    case AVH_RECON_WENO2: return 2;
-    case AVH_RECON_ENO2: return 2;
-    case AVH_RECON_ENO3: return 3;
-    }
-    return 0;
-}
 
 // End of a step, one thread: t += dt, constraint flag, then the writers'
 // save counters (alsfvm/src/simulator/Simulator.cpp:81-85, 180-214;
@@ -169,10 +136,6 @@ EwLauncher ew_for(int eq) {
 
 }  // namespace
 
-struct avh_ctx {
-    int device;
-};
-
 struct avh_solver {
     avh_ctx* ctx = nullptr;
     avh_config cfg{};
@@ -274,7 +237,7 @@ static avh_status enqueue_step(avh_solver* s, KernelTimer* tm = nullptr) {
 
 extern "C" {
 
-const char* avh_last_error(void) { return g_last_error.c_str(); }
+const char* avh_last_error(void) { return last_error_slot().c_str(); }
 
 avh_status avh_ctx_create(int device, avh_ctx** ctx) {
     if (!ctx) return fail(AVH_EINVAL, "ctx is null");
@@ -325,13 +288,7 @@ avh_status avh_solver_create(avh_ctx* ctx, const avh_config* cfg, avh_solver** o
     for (int d = 0; d < dim; ++d)
         if (cfg->n[d] < ng)
             return fail(AVH_EINVAL, "too few cells: every active axis needs at least as many cells as ghost cells");
-    StageLauncher st = nullptr;
-    switch (cfg->equation) {
-    case AVH_EQ_BURGERS: st = get_launcher_burgers(cfg->reconstruction, cfg->flux, dim); break;
-    case AVH_EQ_EULER1: st = get_launcher_euler1(cfg->reconstruction, cfg->flux, dim); break;
-    case AVH_EQ_EULER2: st = get_launcher_euler2(cfg->reconstruction, cfg->flux, dim); break;
-    case AVH_EQ_EULER3: st = get_launcher_euler3(cfg->reconstruction, cfg->flux, dim); break;
-    }
+    const StageLauncher st = launchers_for(cfg->equation, cfg->reconstruction, cfg->flux, dim).stage;
     if (!st)
         return fail(AVH_EINVAL,
                     "unsupported combination of equation, flux and grid dimension (burgers: godunov; "
@@ -478,6 +435,12 @@ avh_status avh_add_fixed_interval_adjuster(avh_solver* s, double interval, doubl
     s->h.adj_interval[i] = interval;
     s->h.adj_start[i] = start;
     s->h.adj_saved[i] = write_initial ? 0 : 1;
+    // the drivers' callWriters() at the current time (alsvinncli/src/main.cpp:210-213)
+    if (s->h.t >= (double)s->h.adj_saved[i] * interval + start) {
+        s->h.adj_saved[i] += 1;
+        s->h.saves_pending += 1;
+        recompute_active(s->h);
+    }
     return push(s);
 }
 

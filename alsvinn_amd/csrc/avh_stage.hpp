@@ -227,7 +227,14 @@ struct StageKernel {
         }
     }
 
-    __device__ static void run(const StageParams& P) {
+    __device__ static void run(const StageParams& P) { run_tile(P, tile_id_xcd(blockIdx.x, gridDim.x)); }
+
+    // number of tiles covering a grid of n cells
+    __host__ __device__ static unsigned int tiles(const int* n) {
+        return (unsigned int)(((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ));
+    }
+
+    __device__ static void run_tile(const StageParams& P, const unsigned int tile) {
         __shared__ double lds[NV * LT + NV * NI_MAX];
         __shared__ double s_ws[3][NT / 64];
         double* su = lds;
@@ -238,8 +245,6 @@ struct StageKernel {
 
         const int n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
         const int ntx = (n0 + TX - 1) / TX, nty = (n1 + TY - 1) / TY;
-        const unsigned int nb = gridDim.x;
-        const unsigned int tile = tile_id_xcd(blockIdx.x, nb);
         const int tx = tile % ntx;
         const int ty = (tile / ntx) % nty;
         const int tz = tile / (ntx * nty);

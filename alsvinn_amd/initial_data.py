@@ -33,27 +33,39 @@ def conserved_from_primitive(rho, u, p, gamma=1.4):
 
 
 def kh_perturbation(x, a, b):
-    """delta(x) = 0.01 * sum_{i=1..10} a_i cos(2 pi (i x + b_i)), a normalised to sum 1."""
+    """delta(x) = 0.01 * sum_{i=1..10} a_i cos(2 pi (i x + b_i)) with a
+    normalised to sum 1 (a sum below 1e-10 counts as 1, as in
+    examples/kelvinhelmholtz/kelvinhelmholtz.py:8-14)."""
     a = np.asarray(a, dtype=np.float64)
-    a = a / a.sum()
+    norm = a.sum()
+    if abs(norm) < 1e-10:
+        norm = 1.0
     out = np.zeros_like(x)
     for i in range(len(a)):
         out = out + a[i] * np.cos(2 * np.pi * ((i + 1) * x + b[i]))
-    return 0.01 * out
+    return 0.01 * out / norm
+
+
+def kh_interfaces(n, params):
+    """Band interfaces at the x midpoints: (lower, upper) = (0.25 + d0(x),
+    0.75 + d1(x)).  The 40 parameters follow the reference example's layout
+    (examples/kelvinhelmholtz/kelvinhelmholtz.py:2-6): a[0:10] amplitudes and
+    a[20:30] phases of the upper interface, a[10:20] and a[30:40] of the
+    lower one."""
+    params = np.asarray(params, dtype=np.float64)
+    x = midpoints(n)
+    lo = 0.25 + kh_perturbation(x, params[10:20], params[30:40])
+    hi = 0.75 + kh_perturbation(x, params[0:10], params[20:30])
+    return lo, hi
 
 
 def kelvin_helmholtz_2d(n, params, gamma=1.4):
     """2D KH on the unit square: rho=2, ux=-0.5 inside the band
-    0.25+d0(x) < y < 0.75+d1(x), else rho=1, ux=0.5; uy=0, p=2.5.
-    params = 40 numbers: a0 (10), a1 (10), b0 (10), b1 (10)."""
-    params = np.asarray(params, dtype=np.float64)
-    a0, a1, b0, b1 = params[0:10], params[10:20], params[20:30], params[30:40]
-    x = midpoints(n)
+    0.25+d0(x) < y < 0.75+d1(x), else rho=1, ux=0.5; uy=0, p=2.5."""
+    lo, hi = kh_interfaces(n, params)
     y = midpoints(n)
-    X, Y = np.meshgrid(x, y)  # (ny, nx)
-    d0 = kh_perturbation(X, a0, b0)
-    d1 = kh_perturbation(X, a1, b1)
-    band = (Y > 0.25 + d0) & (Y < 0.75 + d1)
+    Y = y[:, None]
+    band = (Y > lo[None, :]) & (Y < hi[None, :])
     rho = np.where(band, 2.0, 1.0)
     ux = np.where(band, -0.5, 0.5)
     uy = np.zeros_like(rho)

@@ -22,11 +22,11 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     # the bench line
-    "kh2d": dict(equation="euler2", n=(1024, 1024, 1), nv=4, cfl=0.4,
+    "kh2d": dict(equation="euler2", n=(1024, 1024, 1), nv=4, cfl=0.4, profile="kh2d_1024",
                  desc="2D compressible Euler Kelvin-Helmholtz 1024x1024, periodic, HLL3 + WENO2 + RK3-TVD, "
                       "CFL 0.40, gamma 1.4, FP64"),
     # 3D KH, working set beyond the 256 MB MALL
-    "kh3d": dict(equation="euler3", n=(256, 256, 256), nv=5, cfl=0.3,
+    "kh3d": dict(equation="euler3", n=(256, 256, 256), nv=5, cfl=0.3, profile="kh3d_256",
                  desc="3D compressible Euler Kelvin-Helmholtz 256^3, periodic, HLL3 + WENO2 + RK3-TVD, "
                       "CFL 0.30, gamma 1.4, FP64"),
 }
@@ -149,7 +149,7 @@ def main():
 
     traffic = None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                         "stage_kernel_%s_traffic.json" % args.workload)
+                         "stage_kernel_%s_traffic.json" % WORKLOADS[args.workload]["profile"])
     if os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get("bytes_per_launch")

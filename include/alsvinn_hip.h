@@ -169,6 +169,62 @@ void* avh_stream(avh_solver* solver);
  * host_interior_out and the per-direction max wave speeds to ws[3]. */
 avh_status avh_compute_flux(avh_solver* solver, double* host_interior_out, double ws[3]);
 
+/* ------------------------------------------------------------------
+ * Batched Monte Carlo (alsuq on the hip platform).
+ *
+ * A batch holds up to 4 resolution groups of samples of one configuration
+ * (avh_config.n is ignored; each group has its own grid).  Every launch
+ * advances all samples of all groups by one RK stage; each sample keeps its
+ * own time, dt, save counters, constraint flag and loop test, as the
+ * reference's Runner advances one Simulator per sample
+ * (alsuq/src/run/Runner.cpp:39-90).  Statistics (meanvar) are accumulated on
+ * the device at the statistics save times of FixedIntervalStatistics
+ * (alsuq/src/stats/FixedIntervalStatistics.cpp:33-86) for the conserved
+ * variables (MeanVariance, alsuq/src/stats/MeanVariance.cpp:30-67) and for
+ * rho, u, p.  States are interior-only SoA, nvars * n[0]*n[1]*n[2] doubles. */
+typedef struct avh_batch avh_batch;
+typedef struct avh_group {
+    int n[3];      /* grid of this resolution group */
+    int nsamples;  /* samples in the group */
+} avh_group;
+
+/* stats_saves = the statistics' numberOfSaves (0: no statistics); the
+ * statistics save interval is t_end / stats_saves (alsuq/src/config/
+ * Setup.cpp:270-285).  stats_write_initial records the initial state as
+ * save 0. */
+avh_status avh_batch_create(avh_ctx* ctx, const avh_config* cfg, int ngroups, const avh_group* groups,
+                            double t_end, int stats_saves, int stats_write_initial, avh_batch** batch);
+avh_status avh_batch_destroy(avh_batch* batch);
+/* A further FixedIntervalWriter adjuster on every sample (the fvm
+ * <writer numberOfSaves>, alsfvm/src/config/SimulatorSetup.cpp:222-228). */
+avh_status avh_batch_add_fixed_interval_adjuster(avh_batch* batch, double interval, double start_time,
+                                                 int write_initial);
+avh_status avh_batch_set_state(avh_batch* batch, int group, int sample, const double* host_interior);
+avh_status avh_batch_get_state(avh_batch* batch, int group, int sample, double* host_interior);
+/* 2D Kelvin-Helmholtz initial data on the device from per-sample band
+ * interfaces: host_interfaces[s][0][x] = lower interface y, [s][1][x] =
+ * upper interface y, at the x cell midpoints (euler2 only). */
+avh_status avh_batch_set_kh2d(avh_batch* batch, int group, const double* host_interfaces);
+/* Advance all samples until each reaches t_end (or violates its
+ * constraints), at most max_steps launches of full steps, synchronising
+ * every chunk steps.  active_left = samples still running. */
+avh_status avh_batch_advance(avh_batch* batch, int64_t max_steps, int chunk, int64_t* steps_done,
+                             int64_t* active_left);
+avh_status avh_batch_get_progress(avh_batch* batch, int group, int sample, avh_progress* progress);
+/* Mean and population variance (M2/M) of statistics save `save`;
+ * primitive = 0: conserved variables, 1: rho, u..., p.  count = samples. */
+avh_status avh_batch_get_statistics(avh_batch* batch, int group, int save, int primitive, double* mean,
+                                    double* variance, int64_t* count);
+
+/* Sample parameters as the reference's STLMersenne + Uniform generate them
+ * (alsuq/src/generator/STLMersenne.cpp:25-85, alsuq/src/distribution/
+ * Uniform.cpp:27-34, alsuq/src/samples/SampleGenerator.cpp:29-56): one
+ * default-seeded std::mt19937_64, uniform_real_distribution<double>.
+ * out[(s - sample_start)][...] holds the parameters of sample s in the
+ * given order, each dims[p] long, scaled to [lower[p], upper[p]). */
+avh_status avh_uq_generate(int nparams, const int* dims, const double* lower, const double* upper,
+                           int64_t sample_start, int64_t nsamples, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -900,9 +900,14 @@ int oracle_sim_get_state(void* h, double* ghosted) {
     return 0;
 }
 
+/* Registers a FixedIntervalWriter and performs the drivers' callWriters()
+ * at the current time for it (alsvinncli/src/main.cpp:210-213,
+ * alsuq/src/run/Runner.cpp:68): with writeInitialTimestep the initial state
+ * is a save, numberSaved becomes 1 before the first step. */
 int oracle_sim_add_fixed_interval(void* h, double interval, double start, int write_initial) {
     auto s = static_cast<Simulator*>(h);
     s->adjusters.push_back(FixedInterval{interval, start, (size_t)(write_initial ? 0 : 1)});
+    if (s->adjusters.back().write(s->time)) s->saves++;
     return 0;
 }
 

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     text = open(os.path.join(ROOT, "include", "alsvinn_hip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(avh_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(avh_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_loads():
@@ -34,7 +34,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_exports_via_nm():
     out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True, text=True).stdout
-    exported = set(re.findall(r"\bT (avh_[a-z_]+)", out))
+    exported = set(re.findall(r"\bT (avh_[a-z0-9_]+)", out))
     assert set(header_functions()) <= exported
 
 

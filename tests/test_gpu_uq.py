@@ -1,0 +1,137 @@
+"""Batched Monte Carlo on the GPU against per-sample oracle runs.
+
+Each sample of a batch must equal a single oracle run of that sample bit for
+bit (state, step count, time).  Statistics are Welford on the device; they
+are compared with the reference's naive sums (sum u, sum u^2 in sample
+order, then /M and M2/M - mean^2) within the cancellation error of the
+naive formula: |var_w - var_naive| <= 64 * M * eps * (mean^2 + var)."""
+import sys
+
+import numpy as np
+import pytest
+
+import alsvinn_amd
+from alsvinn_amd import BatchRunner, generate_parameters
+from alsvinn_amd.initial_data import kelvin_helmholtz_2d
+from tests.cases import smooth_state
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def naive_stats(finals):
+    m = len(finals)
+    s1 = np.zeros_like(finals[0])
+    s2 = np.zeros_like(finals[0])
+    for u in finals:
+        s1 = s1 + u
+        s2 = s2 + u * u
+    mean = s1 / m
+    return mean, s2 / m - mean * mean
+
+
+def check_stats(mean, var, finals):
+    m = len(finals)
+    mn, vn = naive_stats(finals)
+    np.testing.assert_allclose(mean, mn, rtol=1e-13, atol=1e-14)
+    bound = 64 * m * EPS * (mn * mn + np.abs(vn)) + 1e-300
+    assert np.all(np.abs(var - vn) <= bound), float(np.max(np.abs(var - vn) / bound))
+
+
+@pytest.mark.parametrize("integ", ["rungekutta3", "rungekutta2", "forwardeuler"])
+def test_kh_batch_two_resolutions_matches_single_runs(integ):
+    from oracle import pyoracle
+    groups = [((32, 32, 1), 5), ((16, 16, 1), 3)]
+    T = 0.04
+    params = generate_parameters([("a", 40, 0.0, 1.0)], 0, 8)["a"]
+    recon = "weno2" if integ != "forwardeuler" else "none"
+    cfl = 0.4 if integ != "forwardeuler" else 0.8
+    _say("case KH batch", integ, groups)
+    br = BatchRunner("euler2", recon, "hll3", integ, "periodic", groups, T, cfl=cfl, stats_saves=1)
+    br.set_kelvin_helmholtz(0, params[:5])
+    br.set_kelvin_helmholtz(1, params[5:])
+    # device KH fill equals the host initial data
+    for g, (n, m) in enumerate(groups):
+        for s in range(m):
+            np.testing.assert_array_equal(br.get_state(g, s), kelvin_helmholtz_2d(n[0], params[(0 if g == 0 else 5) + s]))
+    steps, left = br.run()
+    assert left == 0
+    k = 0
+    for g, (n, m) in enumerate(groups):
+        finals = []
+        for s in range(m):
+            o = pyoracle.Simulator("euler2", recon, "hll3", integ, "periodic", n, cfl=cfl)
+            o.set_interior(kelvin_helmholtz_2d(n[0], params[k]))
+            o.add_fixed_interval(T)  # FixedIntervalStatistics, numberOfSaves = 1
+            o.run(T)
+            p = br.progress(g, s)
+            assert p.steps == o.steps and p.time == o.time == T
+            got = br.get_state(g, s)
+            np.testing.assert_array_equal(got, o.get_interior())
+            finals.append(o.get_interior())
+            k += 1
+        mean, var, cnt = br.statistics(g, 0)
+        assert cnt == m
+        check_stats(mean, var, finals)
+        # rho, u, p statistics
+        prim = []
+        for u in finals:
+            rho = u[0]
+            ux, uy = u[1] / rho, u[2] / rho
+            pr = 0.4 * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / rho)
+            prim.append(np.stack([rho, ux, uy, pr]))
+        pm, pv, _ = br.statistics(g, 0, primitive=True)
+        check_stats(pm, pv, prim)
+
+
+def test_burgers_1d_batch_with_writer_adjuster_and_initial_statistics():
+    """1D burgers batch; an extra fvm writer adjuster (10 saves) and the
+    statistics recorded at t = 0 and at T."""
+    from oracle import pyoracle
+    N, M, T = 96, 6, 0.2
+    br = BatchRunner("burgers", "weno2", "godunov", "rungekutta3", "periodic", [((N, 1, 1), M)], T, cfl=0.45,
+                     stats_saves=1, stats_write_initial=True)
+    br.add_fixed_interval_writer(T / 10)
+    inits = []
+    for s in range(M):
+        q = smooth_state("burgers", (N,), seed=100 + s, jump=(s % 2 == 0))
+        inits.append(q)
+        br.set_state(0, s, q)
+    br.run()
+    finals = []
+    for s in range(M):
+        o = pyoracle.Simulator("burgers", "weno2", "godunov", "rungekutta3", "periodic", (N, 1, 1), cfl=0.45)
+        o.set_interior(inits[s])
+        o.add_fixed_interval(T, 0.0, True)
+        o.add_fixed_interval(T / 10)
+        o.run(T)
+        np.testing.assert_array_equal(br.get_state(0, s), o.get_interior())
+        assert br.progress(0, s).steps == o.steps
+        finals.append(o.get_interior())
+    m0, v0, c0 = br.statistics(0, 0)
+    assert c0 == M
+    check_stats(m0, v0, inits)
+    m1, v1, c1 = br.statistics(0, 1)
+    assert c1 == M
+    check_stats(m1, v1, finals)
+
+
+def test_constant_fields_statistics_are_parameter_moments():
+    """A constant state is steady, so each sample keeps its parameter value;
+    the statistics equal the parameters' mean and population variance."""
+    M = 64
+    x = generate_parameters([("u", 1, 0.0, 1.0)], 0, M)["u"].ravel()
+    br = BatchRunner("burgers", "weno2", "godunov", "rungekutta3", "periodic", [((40, 1, 1), M)], 0.01,
+                     cfl=0.45, stats_saves=1)
+    for s in range(M):
+        br.set_state(0, s, np.full((1, 1, 1, 40), x[s]))
+    br.run()
+    mean, var, cnt = br.statistics(0, 0)
+    assert cnt == M
+    np.testing.assert_allclose(mean.ravel(), np.full(40, x.mean()), rtol=1e-13)
+    np.testing.assert_allclose(var.ravel(), np.full(40, x.var()), rtol=1e-12)
