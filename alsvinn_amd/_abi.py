@@ -32,6 +32,7 @@ SYMBOLS = [
     "avh_set_end_time",
     "avh_enqueue_steps",
     "avh_synchronize",
+    "avh_set_graph_steps",
     "avh_time_step_kernels",
     "avh_stream",
     "avh_compute_flux",
@@ -43,6 +44,8 @@ SYMBOLS = [
     "avh_batch_set_kh2d",
     "avh_batch_advance",
     "avh_batch_get_progress",
+    "avh_batch_set_graph_steps",
+    "avh_batch_time_step_kernels",
     "avh_batch_get_statistics",
     "avh_uq_generate",
 ]
@@ -122,6 +125,9 @@ def load():
         "avh_get_progress": [vp, P(AvhProgress)],
         "avh_set_end_time": [vp, d],
         "avh_enqueue_steps": [vp, i64],
+        "avh_set_graph_steps": [vp, ctypes.c_int],
+        "avh_batch_set_graph_steps": [vp, ctypes.c_int],
+        "avh_batch_time_step_kernels": [vp, i64, P(d), P(ctypes.c_int)],
         "avh_synchronize": [vp],
         "avh_compute_flux": [vp, vp, P(d)],
         "avh_time_step_kernels": [vp, i64, P(d), P(ctypes.c_int)],

@@ -149,6 +149,8 @@ struct avh_solver {
     hipStream_t stream = nullptr;
     StageLauncher stage = nullptr;
     EwLauncher ew = nullptr;
+    int graph_steps = 16;  // steps per captured hipGraph (0: ordinary launches)
+    StepGraph graph;
 };
 
 static avh_status use_device(const avh_solver* s) {
@@ -171,21 +173,9 @@ static void recompute_active(Control& h) {
     h.active = (!(h.t >= h.t_end) && !h.violated && !(h.pause_on_save && h.saves_pending > 0)) ? 1 : 0;
 }
 
-// Optional per-launch event timing (avh_time_step_kernels): an event after
-// every launch of every step, all recorded before one synchronisation.
-struct KernelTimer {
-    std::vector<hipEvent_t> ev;
-    size_t count = 0;
-};
+static avh_status mark(avh_solver* s, EventMarks* t) { return avh::mark(t, s->stream); }
 
-static avh_status mark(avh_solver* s, KernelTimer* t) {
-    if (!t) return AVH_OK;
-    if (t->count >= t->ev.size()) return fail(AVH_EINVAL, "kernel timer overflow");
-    AVH_HIP(hipEventRecord(t->ev[t->count++], s->stream));
-    return AVH_OK;
-}
-
-static avh_status enqueue_step(avh_solver* s, KernelTimer* tm = nullptr) {
+static avh_status enqueue_step(avh_solver* s, EventMarks* tm = nullptr) {
     StageParams P;
     P.ctrl = s->ctrl;
     P.ncells = s->ncells;
@@ -235,6 +225,10 @@ static avh_status enqueue_step(avh_solver* s, KernelTimer* tm = nullptr) {
     return AVH_OK;
 }
 
+static avh_status enqueue_steps(avh_solver* s, int64_t n) {
+    return enqueue_steps_graphed(s->graph, s->graph_steps, s->stream, n, [s]() { return enqueue_step(s); });
+}
+
 extern "C" {
 
 const char* avh_last_error(void) { return last_error_slot().c_str(); }
@@ -261,6 +255,7 @@ avh_status avh_solver_destroy(avh_solver* s) {
     if (!s) return AVH_OK;
     hipSetDevice(s->ctx->device);
     if (s->stream) hipStreamSynchronize(s->stream);
+    s->graph.reset();
     hipFree(s->A);
     hipFree(s->B);
     hipFree(s->C);
@@ -458,8 +453,16 @@ avh_status avh_enqueue_steps(avh_solver* s, int64_t steps) {
     if (!s) return fail(AVH_EINVAL, "null argument");
     avh_status r = use_device(s);
     if (r) return r;
-    for (int64_t i = 0; i < steps; ++i)
-        if ((r = enqueue_step(s))) return r;
+    return enqueue_steps(s, steps);
+}
+
+avh_status avh_set_graph_steps(avh_solver* s, int steps) {
+    if (!s || steps < 0 || steps > 4096) return fail(AVH_EINVAL, "bad argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    s->graph.reset();
+    s->graph_steps = steps;
     return AVH_OK;
 }
 
@@ -480,29 +483,11 @@ avh_status avh_time_step_kernels(avh_solver* s, int64_t steps, double* ms, int* 
     // launches per step: the stages (plus the element-wise pass of forward
     // euler / rk2) and the end-of-step kernel
     const int nk = s->cfg.integrator == AVH_INT_FORWARDEULER ? 3 : 4;
-    KernelTimer t;
-    t.ev.resize((size_t)steps * (nk + 1));
-    size_t made = 0;
-    for (; made < t.ev.size(); ++made)
-        if (hipEventCreate(&t.ev[made]) != hipSuccess) break;
-    if (made < t.ev.size()) {
-        for (size_t i = 0; i < made; ++i) hipEventDestroy(t.ev[i]);
-        return fail(AVH_EHIP, "hipEventCreate failed");
-    }
-    for (int64_t k = 0; k < steps && r == AVH_OK; ++k) r = enqueue_step(s, &t);
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (!r && hipEventSynchronize(t.ev[t.count - 1]) != hipSuccess) r = fail(AVH_EHIP, "hipEventSynchronize failed");
-    if (!r) {
-        for (int64_t k = 0; k < steps; ++k)
-            for (int i = 0; i < nk; ++i) {
-                float f = 0;
-                hipEventElapsedTime(&f, t.ev[k * (nk + 1) + i], t.ev[k * (nk + 1) + i + 1]);
-                acc[i] += f;
-            }
-    }
-    for (auto& e : t.ev) hipEventDestroy(e);
-    if (r) return r;
-    for (int i = 0; i < nk; ++i) ms[i] = acc[i] / (double)steps;
+    EventMarks t;
+    if ((r = t.create((size_t)steps * (nk + 1)))) return r;
+    for (int64_t k = 0; k < steps; ++k)
+        if ((r = enqueue_step(s, &t))) return r;
+    if ((r = t.average(steps, nk, ms))) return r;
     *kernels = nk;
     return AVH_OK;
 }
@@ -586,8 +571,7 @@ avh_status avh_advance(avh_solver* s, double t_end, int64_t max_steps, int chunk
     while (done < max_steps && s->h.active) {
         const int64_t k = (max_steps - done) < chunk ? (max_steps - done) : chunk;
         const int64_t before = s->h.steps;
-        for (int64_t i = 0; i < k; ++i)
-            if ((r = enqueue_step(s))) return r;
+        if ((r = enqueue_steps(s, k))) return r;
         if ((r = pull(s))) return r;
         done += s->h.steps - before;
         if (s->h.steps - before < k) break;  // the device loop ended inside the chunk

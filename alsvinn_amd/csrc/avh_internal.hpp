@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/alsvinn_hip.h"
 #include "avh_stage_inst.hpp"
@@ -58,6 +60,108 @@ inline Launchers launchers_for(int equation, int recon, int flux, int dim) {
     case AVH_EQ_EULER3: return get_launchers_euler3(recon, flux, dim);
     }
     return Launchers();
+}
+
+// Per-launch timing: an event after every launch of every step, all
+// recorded before one synchronisation; nk launches and nk + 1 marks per step.
+struct EventMarks {
+    std::vector<hipEvent_t> ev;
+    size_t count = 0;
+    EventMarks() = default;
+    EventMarks(const EventMarks&) = delete;
+    EventMarks& operator=(const EventMarks&) = delete;
+    ~EventMarks() {
+        for (auto e : ev) hipEventDestroy(e);
+    }
+    avh_status create(size_t n) {
+        ev.reserve(n);
+        for (size_t i = 0; i < n; ++i) {
+            hipEvent_t e;
+            AVH_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        return AVH_OK;
+    }
+    avh_status mark(hipStream_t st) {
+        if (count >= ev.size()) return fail(AVH_EINVAL, "kernel timer overflow");
+        AVH_HIP(hipEventRecord(ev[count++], st));
+        return AVH_OK;
+    }
+    // ms[i] = mean duration of launch i over the steps
+    avh_status average(int64_t steps, int nk, double* ms) {
+        if (count != (size_t)steps * (nk + 1)) return fail(AVH_EINVAL, "kernel timer: unexpected launch count");
+        AVH_HIP(hipEventSynchronize(ev[count - 1]));
+        for (int i = 0; i < nk; ++i) ms[i] = 0;
+        for (int64_t k = 0; k < steps; ++k)
+            for (int i = 0; i < nk; ++i) {
+                float f = 0;
+                AVH_HIP(hipEventElapsedTime(&f, ev[k * (nk + 1) + i], ev[k * (nk + 1) + i + 1]));
+                ms[i] += f;
+            }
+        for (int i = 0; i < nk; ++i) ms[i] /= (double)steps;
+        return AVH_OK;
+    }
+};
+
+inline avh_status mark(EventMarks* t, hipStream_t st) { return t ? t->mark(st) : AVH_OK; }
+
+// A chunk of time steps captured once into a hipGraph and replayed. The step
+// launches take fixed buffer pointers and read everything that changes (dt,
+// time, save counters, the active flag) from the device control blocks, so a
+// captured chunk stays valid for the solver's lifetime. No memset nodes are
+// recorded: the end-of-step kernel clears what must start at zero.
+struct StepGraph {
+    int steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void reset() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        steps = 0;
+    }
+    ~StepGraph() { reset(); }
+};
+
+// Enqueues n steps: whole chunks of graph_steps as graph launches (captured
+// on first use), the remainder as ordinary launches. graph_steps < 2 turns
+// graphs off.
+template <class Step>
+avh_status enqueue_steps_graphed(StepGraph& G, int graph_steps, hipStream_t st, int64_t n, Step&& step) {
+    if (graph_steps >= 2 && n >= graph_steps) {
+        if (G.steps != graph_steps || !G.exec) {
+            G.reset();
+            AVH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            avh_status r = AVH_OK;
+            for (int i = 0; i < graph_steps && r == AVH_OK; ++i) r = step();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(st, &g);
+            if (r != AVH_OK) {
+                if (g) hipGraphDestroy(g);
+                return r;
+            }
+            if (e != hipSuccess) return fail(AVH_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            hipGraphExec_t x = nullptr;
+            const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            if (e2 != hipSuccess) {
+                hipGraphDestroy(g);
+                return fail(AVH_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+            }
+            G.graph = g;
+            G.exec = x;
+            G.steps = graph_steps;
+        }
+        while (n >= graph_steps) {
+            AVH_HIP(hipGraphLaunch(G.exec, st));
+            n -= graph_steps;
+        }
+    }
+    for (; n > 0; --n) {
+        avh_status r = step();
+        if (r != AVH_OK) return r;
+    }
+    return AVH_OK;
 }
 
 }  // namespace avh

@@ -186,6 +186,8 @@ struct avh_batch {
     BatchLauncher stage = nullptr;
     BatchEw ew = nullptr;
     BatchStats stats = nullptr;
+    int graph_steps = 8;  // steps per captured hipGraph (0: ordinary launches)
+    StepGraph graph;
 };
 
 namespace {
@@ -194,6 +196,7 @@ void free_batch(avh_batch* b) {
     if (!b) return;
     hipSetDevice(b->ctx->device);
     if (b->stream) hipStreamSynchronize(b->stream);
+    b->graph.reset();
     for (int i = 0; i < b->ngroups; ++i) {
         GroupHost& G = b->g[i];
         hipFree(G.A);
@@ -270,7 +273,17 @@ StatsParams stats_params(const avh_batch* b, unsigned int* nblocks) {
     return S;
 }
 
-avh_status enqueue_batch_step(avh_batch* b) {
+// Launches per batch step: the stages (plus forward euler's / rk2's
+// element-wise pass), the end-of-step kernel and, with statistics, the
+// statistics kernel.
+int batch_launches_per_step(const avh_batch* b) {
+    const int stages = b->cfg.integrator == AVH_INT_FORWARDEULER ? 2 : 3;
+    return stages + 1 + (b->nsaves > 0 ? 1 : 0);
+}
+
+avh_status enqueue_batch_step(avh_batch* b, EventMarks* tm = nullptr) {
+    avh_status r;
+    if ((r = mark(tm, b->stream))) return r;
     unsigned int nb = 0, nbe = 0;
     auto set_ptrs = [&](BatchParams& P, int which_in, int which_u0, int which_out) {
         for (int i = 0; i < b->ngroups; ++i) {
@@ -284,26 +297,32 @@ avh_status enqueue_batch_step(avh_batch* b) {
     BatchParams P = batch_params(b, MODE_L, false, &nb);
     set_ptrs(P, A, A, B);
     AVH_HIP(b->stage(P, nb, b->stream));
+    if ((r = mark(tm, b->stream))) return r;
     if (b->cfg.integrator == AVH_INT_FORWARDEULER) {
         BatchParams E = batch_params(b, MODE_L, true, &nbe);
         set_ptrs(E, B, A, A);
         b->ew(E, nbe, 1, b->stream);
         AVH_HIP(hipGetLastError());
+        if ((r = mark(tm, b->stream))) return r;
     } else if (b->cfg.integrator == AVH_INT_RUNGEKUTTA2) {
         BatchParams E = batch_params(b, MODE_L, true, &nbe);
         set_ptrs(E, B, A, C);
         b->ew(E, nbe, 0, b->stream);
         AVH_HIP(hipGetLastError());
+        if ((r = mark(tm, b->stream))) return r;
         P = batch_params(b, MODE_RK2_2, false, &nb);
         set_ptrs(P, C, A, A);
         AVH_HIP(b->stage(P, nb, b->stream));
+        if ((r = mark(tm, b->stream))) return r;
     } else {
         P = batch_params(b, MODE_RK3_2, false, &nb);
         set_ptrs(P, B, A, C);
         AVH_HIP(b->stage(P, nb, b->stream));
+        if ((r = mark(tm, b->stream))) return r;
         P = batch_params(b, MODE_RK3_3, false, &nb);
         set_ptrs(P, C, A, A);
         AVH_HIP(b->stage(P, nb, b->stream));
+        if ((r = mark(tm, b->stream))) return r;
     }
     unsigned int nbs = 0;
     StatsParams S = stats_params(b, &nbs);
@@ -311,9 +330,11 @@ avh_status enqueue_batch_step(avh_batch* b) {
     for (int i = 0; i < b->ngroups; ++i) maxs = std::max(maxs, std::min(1024, (b->g[i].nsamples + 63) / 64 * 64));
     hipLaunchKernelGGL(batch_finish_kernel, dim3(b->ngroups), dim3(maxs), 0, b->stream, S);
     AVH_HIP(hipGetLastError());
+    if ((r = mark(tm, b->stream))) return r;
     if (b->nsaves > 0) {
         b->stats(S, nbs, b->stream);
         AVH_HIP(hipGetLastError());
+        if ((r = mark(tm, b->stream))) return r;
     }
     return AVH_OK;
 }
@@ -531,10 +552,9 @@ avh_status avh_batch_advance(avh_batch* b, int64_t max_steps, int chunk, int64_t
     int64_t done = 0, left = 0;
     while (done < max_steps) {
         const int64_t k = std::min<int64_t>(chunk, max_steps - done);
-        for (int64_t i = 0; i < k; ++i) {
-            avh_status r = enqueue_batch_step(b);
-            if (r) return r;
-        }
+        avh_status r = enqueue_steps_graphed(b->graph, b->graph_steps, b->stream, k,
+                                             [b]() { return enqueue_batch_step(b); });
+        if (r) return r;
         done += k;
         AVH_HIP(hipStreamSynchronize(b->stream));
         left = 0;
@@ -547,6 +567,29 @@ avh_status avh_batch_advance(avh_batch* b, int64_t max_steps, int chunk, int64_t
     }
     if (steps_done) *steps_done = done;
     if (active_left) *active_left = left;
+    return AVH_OK;
+}
+
+avh_status avh_batch_set_graph_steps(avh_batch* b, int steps) {
+    if (!b || steps < 0 || steps > 4096) return fail(AVH_EINVAL, "bad argument");
+    AVH_HIP(hipSetDevice(b->ctx->device));
+    AVH_HIP(hipStreamSynchronize(b->stream));
+    b->graph.reset();
+    b->graph_steps = steps;
+    return AVH_OK;
+}
+
+avh_status avh_batch_time_step_kernels(avh_batch* b, int64_t steps, double* ms, int* kernels) {
+    if (!b || !ms || !kernels || steps < 1 || steps > 100000) return fail(AVH_EINVAL, "bad argument");
+    AVH_HIP(hipSetDevice(b->ctx->device));
+    const int nk = batch_launches_per_step(b);
+    EventMarks t;
+    avh_status r = t.create((size_t)steps * (nk + 1));
+    if (r) return r;
+    for (int64_t k = 0; k < steps; ++k)
+        if ((r = enqueue_batch_step(b, &t))) return r;
+    if ((r = t.average(steps, nk, ms))) return r;
+    *kernels = nk;
     return AVH_OK;
 }
 

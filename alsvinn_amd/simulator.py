@@ -222,6 +222,10 @@ class Simulator:
     def synchronize(self):
         _abi.check(_abi.load().avh_synchronize(self._h))
 
+    def set_graph_steps(self, steps):
+        """Steps per captured hipGraph (0: ordinary launches)."""
+        _abi.check(_abi.load().avh_set_graph_steps(self._h, int(steps)))
+
     def stream(self):
         return _abi.load().avh_stream(self._h)
 

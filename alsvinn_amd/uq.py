@@ -119,6 +119,18 @@ class BatchRunner:
                                                  ctypes.byref(left)))
         return done.value, left.value
 
+    def set_graph_steps(self, steps):
+        """Steps per captured hipGraph (0: ordinary launches)."""
+        _abi.check(_abi.load().avh_batch_set_graph_steps(self._h, int(steps)))
+
+    def time_step_kernels(self, steps):
+        """Advance `steps` steps with an event after every launch; returns the
+        mean milliseconds of each launch of a step, in launch order."""
+        ms = (ctypes.c_double * 8)()
+        nk = ctypes.c_int()
+        _abi.check(_abi.load().avh_batch_time_step_kernels(self._h, int(steps), ms, ctypes.byref(nk)))
+        return [ms[i] for i in range(nk.value)]
+
     def progress(self, group, sample):
         p = _abi.AvhProgress()
         _abi.check(_abi.load().avh_batch_get_progress(self._h, int(group), int(sample), ctypes.byref(p)))

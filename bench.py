@@ -32,6 +32,14 @@ WORKLOADS = {
 }
 
 
+# Batched Monte Carlo of the 2D KH problem: 1024 samples at 256^2, 256 at
+# 512^2 and 64 at 128^2 advanced together, on-device mean/variance at T = 2.
+UQ_GROUPS = [((256, 256, 1), 1024), ((512, 512, 1), 256), ((128, 128, 1), 64)]
+UQ_DESC = ("batched Monte Carlo 2D compressible Euler Kelvin-Helmholtz: 1024 samples at 256^2 + 256 at 512^2 "
+           "+ 64 at 128^2 in the same launches, 40 U[0,1] parameters per sample, periodic, HLL3 + WENO2 + "
+           "RK3-TVD, CFL 0.4, gamma 1.4, FP64, on-device Welford mean/variance at T = 2")
+
+
 def algorithmic_bytes_per_cell_step(nv):
     # per RK3 step: stage 1 reads U0, writes L0 (2 arrays); stage 2 reads L0,
     # U0, writes U2 (3); stage 3 reads U2, U0, writes U3 (3): 8 arrays of nv
@@ -79,15 +87,93 @@ def cpu_baseline(wl, seconds_budget=15.0):
     }
 
 
+def run_uq(args):
+    import alsvinn_amd
+    from alsvinn_amd import BatchRunner, generate_parameters
+
+    total = sum(m for _, m in UQ_GROUPS)
+    print("bench: building %d UQ samples" % total, file=sys.stderr, flush=True)
+    params = generate_parameters([("a", 40, 0.0, 1.0)], 0, total)["a"]
+    br = BatchRunner("euler2", "weno2", "hll3", "rungekutta3", "periodic", UQ_GROUPS, 2.0, cfl=0.4, stats_saves=1)
+    k = 0
+    for g, (n, m) in enumerate(UQ_GROUPS):
+        br.set_kelvin_helmholtz(g, params[k:k + m])
+        k += m
+    cells = sum(n[0] * n[1] * m for n, m in UQ_GROUPS)
+    br.run(max_steps=args.warmup, chunk=max(1, args.warmup))
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    done, left = br.run(max_steps=args.steps, chunk=args.steps)
+    el = time.perf_counter() - t0
+    if left != total:
+        raise SystemExit("samples finished or failed inside the timed steps (%d of %d active)" % (left, total))
+    value = cells * args.steps / el
+    launch_ms = br.time_step_kernels(args.profile_steps)
+    stage_ms = launch_ms[:3]
+    sb = stage_bytes_per_cell(4)
+    avg_ms = sum(stage_ms) / 3
+    avg_bytes = cells * sum(sb) / len(sb)
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "batch_stage_kernel_uq_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
+        wl = dict(equation="euler2", n=(256, 256, 1), nv=4, cfl=0.4)
+        cpu = cpu_baseline(wl, args.cpu_seconds)
+        cpu["sample"] = ("one 256^2 sample of the batch: " + cpu["sample"] +
+                         "; samples are independent, so the batch's CPU rate is this per-cell rate")
+    return {
+        "metric": "FP64 cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": UQ_DESC,
+            "cells": cells,
+            "samples": total,
+            "step": "one RK3 time step of every sample (3 batched stage launches, per-sample on-device CFL dt, "
+                    "end-of-step and statistics kernels)",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "avh::batch_stage_kernel (fused reconstruct + flux + RK stage, all samples)",
+            "algorithmic_bytes_per_launch": avg_bytes,
+            "stage_ms": stage_ms,
+            "end_of_step_kernel_ms": launch_ms[3],
+            "statistics_kernel_ms": launch_ms[4] if len(launch_ms) > 4 else None,
+        },
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 600; uq: 100)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 30; uq: 5)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=100)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="kh2d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["uq"], default="kh2d")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (torch copy kernel), to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters against a known byte count")
@@ -109,6 +195,14 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.steps is None:
+        args.steps = 100 if args.workload == "uq" else 600
+    if args.warmup is None:
+        args.warmup = 5 if args.workload == "uq" else 30
+    if args.workload == "uq":
+        print(json.dumps(run_uq(args)), flush=True)
+        return
 
     import alsvinn_amd
 

@@ -151,6 +151,11 @@ avh_status avh_set_end_time(avh_solver* solver, double t_end);
  * synchronisation (for timing and for callers that poll progress). */
 avh_status avh_enqueue_steps(avh_solver* solver, int64_t steps);
 avh_status avh_synchronize(avh_solver* solver);
+/* Steps per captured hipGraph used by avh_enqueue_steps / avh_advance
+ * (default 16; 0 or 1: ordinary launches).  A chunk of steps is captured
+ * once and replayed; results are identical either way.  The reference
+ * launches each kernel separately (alsfvm/src/simulator/Simulator.cpp:191-214). */
+avh_status avh_set_graph_steps(avh_solver* solver, int steps);
 
 /* Run `steps` full time steps with a HIP event pair around every kernel
  * launch on the solver's stream (the hipEvent analogue of the reference's
@@ -211,6 +216,15 @@ avh_status avh_batch_set_kh2d(avh_batch* batch, int group, const double* host_in
 avh_status avh_batch_advance(avh_batch* batch, int64_t max_steps, int chunk, int64_t* steps_done,
                              int64_t* active_left);
 avh_status avh_batch_get_progress(avh_batch* batch, int group, int sample, avh_progress* progress);
+/* Steps per captured hipGraph in avh_batch_advance (default 8; 0 or 1:
+ * ordinary launches). */
+avh_status avh_batch_set_graph_steps(avh_batch* batch, int steps);
+/* Advance every sample `steps` full steps with a HIP event after every
+ * launch; ms_per_launch[k] = average duration of the k-th launch of a step
+ * (k < *kernels_per_step <= 6: stages, element-wise pass, end-of-step
+ * kernel, statistics kernel), as avh_time_step_kernels. */
+avh_status avh_batch_time_step_kernels(avh_batch* batch, int64_t steps, double* ms_per_launch,
+                                       int* kernels_per_step);
 /* Mean and population variance (M2/M) of statistics save `save`;
  * primitive = 0: conserved variables, 1: rho, u..., p.  count = samples. */
 avh_status avh_batch_get_statistics(avh_batch* batch, int group, int save, int primitive, double* mean,
