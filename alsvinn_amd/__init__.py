@@ -4,7 +4,7 @@ The compute path is libalsvinn_hip.so (hand-written HIP kernels for gfx950
 behind the C ABI in include/alsvinn_hip.h); this package is the host-side
 mirror of the reference's Simulator surface on top of it.
 """
-from ._abi import ConstraintError, HipError, LibraryMissing, SYMBOLS, load  # noqa: F401
+from ._abi import ConstraintError, HipError, InvalidArgument, LibraryMissing, SYMBOLS, load  # noqa: F401
 from .simulator import (  # noqa: F401
     BOUNDARIES,
     CONSERVED_NAMES,

@@ -33,6 +33,8 @@ SYMBOLS = [
     "avh_enqueue_steps",
     "avh_synchronize",
     "avh_set_graph_steps",
+    "avh_set_tile_shape",
+    "avh_autotune",
     "avh_time_step_kernels",
     "avh_stream",
     "avh_compute_flux",
@@ -45,6 +47,8 @@ SYMBOLS = [
     "avh_batch_advance",
     "avh_batch_get_progress",
     "avh_batch_set_graph_steps",
+    "avh_batch_set_tile_shape",
+    "avh_batch_autotune",
     "avh_batch_time_step_kernels",
     "avh_batch_get_statistics",
     "avh_uq_generate",
@@ -126,6 +130,10 @@ def load():
         "avh_set_end_time": [vp, d],
         "avh_enqueue_steps": [vp, i64],
         "avh_set_graph_steps": [vp, ctypes.c_int],
+        "avh_set_tile_shape": [vp, ctypes.c_int],
+        "avh_autotune": [vp, P(ctypes.c_int), P(d), P(ctypes.c_int)],
+        "avh_batch_set_tile_shape": [vp, ctypes.c_int],
+        "avh_batch_autotune": [vp, P(ctypes.c_int), P(d), P(ctypes.c_int)],
         "avh_batch_set_graph_steps": [vp, ctypes.c_int],
         "avh_batch_time_step_kernels": [vp, i64, P(d), P(ctypes.c_int)],
         "avh_synchronize": [vp],
@@ -156,6 +164,10 @@ class HipError(RuntimeError):
     """Raised for AVH_EHIP / AVH_ENOMEM / AVH_EINVAL."""
 
 
+class InvalidArgument(ValueError):
+    """AVH_EINVAL: a rejected argument (no such shape, group, sample ...)."""
+
+
 class ConstraintError(RuntimeError):
     """AVH_ECONSTRAINT: the reference throws std::runtime_error from
     Simulator::checkConstraints (alsfvm/src/simulator/Simulator.cpp:180-189)."""
@@ -167,4 +179,6 @@ def check(status):
     msg = load().avh_last_error().decode(errors="replace")
     if status == AVH_ECONSTRAINT:
         raise ConstraintError(msg)
+    if status == AVH_EINVAL:
+        raise InvalidArgument(msg)
     raise HipError("avh status %d: %s" % (status, msg))

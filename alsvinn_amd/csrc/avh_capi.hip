@@ -151,6 +151,7 @@ struct avh_solver {
     EwLauncher ew = nullptr;
     int graph_steps = 16;  // steps per captured hipGraph (0: ordinary launches)
     StepGraph graph;
+    int shape = -1;        // tile shape of the stage kernel; -1: autotune on first use
 };
 
 static avh_status use_device(const avh_solver* s) {
@@ -189,7 +190,7 @@ static avh_status enqueue_step(avh_solver* s, EventMarks* tm = nullptr) {
     P.mode = MODE_L;
     avh_status r;
     if ((r = mark(s, tm))) return r;
-    AVH_HIP(s->stage(P, s->stream));
+    AVH_HIP(s->stage(P, s->shape, s->stream));
     if ((r = mark(s, tm))) return r;
     if (s->cfg.integrator == AVH_INT_FORWARDEULER) {
         s->ew(s->B, s->A, s->A, s->ctrl, s->ncells, EW_FE_FINISH, s->cfg.gamma, s->stream);
@@ -203,20 +204,20 @@ static avh_status enqueue_step(avh_solver* s, EventMarks* tm = nullptr) {
         P.u0 = s->A;
         P.out = s->A;
         P.mode = MODE_RK2_2;
-        AVH_HIP(s->stage(P, s->stream));
+        AVH_HIP(s->stage(P, s->shape, s->stream));
         if ((r = mark(s, tm))) return r;
     } else {
         P.in = s->B;
         P.u0 = s->A;
         P.out = s->C;
         P.mode = MODE_RK3_2;
-        AVH_HIP(s->stage(P, s->stream));
+        AVH_HIP(s->stage(P, s->shape, s->stream));
         if ((r = mark(s, tm))) return r;
         P.in = s->C;
         P.u0 = s->A;
         P.out = s->A;
         P.mode = MODE_RK3_3;
-        AVH_HIP(s->stage(P, s->stream));
+        AVH_HIP(s->stage(P, s->shape, s->stream));
         if ((r = mark(s, tm))) return r;
     }
     hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s->stream, s->ctrl);
@@ -225,7 +226,67 @@ static avh_status enqueue_step(avh_solver* s, EventMarks* tm = nullptr) {
     return AVH_OK;
 }
 
+static TuneKey tune_key(const avh_solver* s) {
+    return TuneKey(0, s->cfg.equation * 16 + s->cfg.reconstruction, s->cfg.flux, s->dim, s->cfg.n[0], s->cfg.n[1],
+                   s->cfg.n[2]);
+}
+
+// Times every tile shape with flux-only launches on the current state (they
+// write the L0 buffer, which stage 1 overwrites, and add to the wave-speed
+// maxima, which are restored) and keeps the fastest.
+static avh_status autotune(avh_solver* s, int* best, double* ms, int* nshapes) {
+    const int ns = num_shapes(s->dim);
+    avh_status r = pull(s);
+    if (r) return r;
+    const Control saved = s->h;
+    StageParams P;
+    P.ctrl = s->ctrl;
+    P.ncells = s->ncells;
+    for (int d = 0; d < 3; ++d) P.n[d] = s->cfg.n[d];
+    P.bc = s->cfg.boundary;
+    P.gamma = s->cfg.gamma;
+    P.in = s->A;
+    P.u0 = s->A;
+    P.out = s->B;
+    P.mode = MODE_FLUX_ONLY;
+    int b = 0;
+    double tmin = 0;
+    for (int k = 0; k < ns; ++k) {
+        double t = 0;
+        if (ns > 1) {
+            r = time_launches(s->stream, 5, [&]() { return s->stage(P, k, s->stream); }, &t);
+            if (r) return r;
+        }
+        if (ms) ms[k] = t;
+        if (k == 0 || t < tmin) {
+            tmin = t;
+            b = k;
+        }
+    }
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    s->h = saved;
+    if ((r = push(s))) return r;
+    if (s->shape != b) s->graph.reset();
+    s->shape = b;
+    tune_store(tune_key(s), b);
+    if (best) *best = b;
+    if (nshapes) *nshapes = ns;
+    return AVH_OK;
+}
+
+static avh_status ensure_shape(avh_solver* s) {
+    if (s->shape >= 0) return AVH_OK;
+    int b;
+    if (tune_lookup(tune_key(s), &b)) {
+        s->shape = b;
+        return AVH_OK;
+    }
+    return autotune(s, nullptr, nullptr, nullptr);
+}
+
 static avh_status enqueue_steps(avh_solver* s, int64_t n) {
+    avh_status r = ensure_shape(s);
+    if (r) return r;
     return enqueue_steps_graphed(s->graph, s->graph_steps, s->stream, n, [s]() { return enqueue_step(s); });
 }
 
@@ -466,6 +527,24 @@ avh_status avh_set_graph_steps(avh_solver* s, int steps) {
     return AVH_OK;
 }
 
+avh_status avh_set_tile_shape(avh_solver* s, int shape) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    if (shape < -1 || shape >= num_shapes(s->dim)) return fail(AVH_EINVAL, "no such tile shape");
+    avh_status r = use_device(s);
+    if (r) return r;
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    s->graph.reset();
+    s->shape = shape;
+    return AVH_OK;
+}
+
+avh_status avh_autotune(avh_solver* s, int* best_shape, double* ms_per_shape, int* nshapes) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    return autotune(s, best_shape, ms_per_shape, nshapes);
+}
+
 avh_status avh_synchronize(avh_solver* s) {
     if (!s) return fail(AVH_EINVAL, "null argument");
     avh_status r = use_device(s);
@@ -483,6 +562,7 @@ avh_status avh_time_step_kernels(avh_solver* s, int64_t steps, double* ms, int* 
     // launches per step: the stages (plus the element-wise pass of forward
     // euler / rk2) and the end-of-step kernel
     const int nk = s->cfg.integrator == AVH_INT_FORWARDEULER ? 3 : 4;
+    if ((r = ensure_shape(s))) return r;
     EventMarks t;
     if ((r = t.create((size_t)steps * (nk + 1)))) return r;
     for (int64_t k = 0; k < steps; ++k)
@@ -541,6 +621,7 @@ avh_status avh_step(avh_solver* s, double* dt_out) {
     s->h.pause_on_save = 0;
     s->h.active = 1;
     if ((r = push(s))) return r;
+    if ((r = ensure_shape(s))) return r;
     if ((r = enqueue_step(s))) return r;
     if ((r = pull(s))) return r;
     s->h.t_end = t_end;
@@ -585,6 +666,7 @@ avh_status avh_compute_flux(avh_solver* s, double* host_out, double ws[3]) {
     if (!s || !host_out || !ws) return fail(AVH_EINVAL, "null argument");
     avh_status r = use_device(s);
     if (r) return r;
+    if ((r = ensure_shape(s))) return r;
     if ((r = pull(s))) return r;
     StageParams P;
     P.ctrl = s->ctrl;
@@ -596,7 +678,7 @@ avh_status avh_compute_flux(avh_solver* s, double* host_out, double ws[3]) {
     P.u0 = s->A;
     P.out = s->B;
     P.mode = MODE_FLUX_ONLY;
-    AVH_HIP(s->stage(P, s->stream));
+    AVH_HIP(s->stage(P, s->shape, s->stream));
     AVH_HIP(hipMemcpyAsync(host_out, s->B, (size_t)s->nv * s->ncells * sizeof(double), hipMemcpyDeviceToHost,
                            s->stream));
     if ((r = pull(s))) return r;

@@ -3,7 +3,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/alsvinn_hip.h"
@@ -161,6 +164,58 @@ avh_status enqueue_steps_graphed(StepGraph& G, int graph_steps, hipStream_t st, 
         avh_status r = step();
         if (r != AVH_OK) return r;
     }
+    return AVH_OK;
+}
+
+// Autotuner cache: the tile shape chosen for a (equation, reconstruction,
+// flux, grid) of single solvers (kind 0) or batches (kind 1, grid = the
+// first group, n[2] = total samples), shared by every solver of the process.
+typedef std::tuple<int, int, int, int, long long, long long, long long> TuneKey;
+
+inline std::mutex& tune_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+inline std::map<TuneKey, int>& tune_cache() {
+    static std::map<TuneKey, int> c;
+    return c;
+}
+
+inline bool tune_lookup(const TuneKey& k, int* shape) {
+    std::lock_guard<std::mutex> g(tune_mutex());
+    auto it = tune_cache().find(k);
+    if (it == tune_cache().end()) return false;
+    *shape = it->second;
+    return true;
+}
+
+inline void tune_store(const TuneKey& k, int shape) {
+    std::lock_guard<std::mutex> g(tune_mutex());
+    tune_cache()[k] = shape;
+}
+
+// Mean milliseconds of `reps` back-to-back calls of launch(), after one
+// untimed call, with HIP events on the launch stream.
+template <class Launch>
+avh_status time_launches(hipStream_t st, int reps, Launch&& launch, double* ms) {
+    hipEvent_t e0, e1;
+    AVH_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return fail(AVH_EHIP, "hipEventCreate failed");
+    }
+    hipError_t e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float f = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(AVH_EHIP, std::string("autotune timing: ") + hipGetErrorString(e));
+    *ms = f / reps;
     return AVH_OK;
 }
 

@@ -152,6 +152,7 @@ struct StageKernel {
     static constexpr int NI_MAX = NI_X > NI_Y ? (NI_X > NI_Z ? NI_X : NI_Z) : (NI_Y > NI_Z ? NI_Y : NI_Z);
     static constexpr int TC = TX * TY * TZ;
     static constexpr int TXC = TX, TYC = TY, TZC = TZ;
+    static constexpr int NTHREADS = NT;
     static constexpr int CPT = (TC + NT - 1) / NT;  // cells per thread in phase 3
     static constexpr int LDS_BYTES = (NV * LT + NV * NI_MAX) * 8;
 

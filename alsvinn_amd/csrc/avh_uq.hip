@@ -129,7 +129,6 @@ struct GroupHost {
     int n[3];
     int nsamples;
     long long ncells;
-    unsigned int tiles;  // stage tiles per sample
     double *A = nullptr, *B = nullptr, *C = nullptr;
     Control* ctrl = nullptr;
     double *mean_c = nullptr, *m2_c = nullptr, *mean_p = nullptr, *m2_p = nullptr;
@@ -188,6 +187,7 @@ struct avh_batch {
     BatchStats stats = nullptr;
     int graph_steps = 8;  // steps per captured hipGraph (0: ordinary launches)
     StepGraph graph;
+    int shape = -1;       // tile shape of the stage kernel; -1: autotune on first use
 };
 
 namespace {
@@ -231,7 +231,7 @@ BatchParams batch_params(const avh_batch* b, int mode, bool ew_blocks, unsigned 
         D.stride = (long long)b->nv * G.ncells;
         D.ncells = G.ncells;
         for (int d = 0; d < 3; ++d) D.n[d] = G.n[d];
-        D.tiles = ew_blocks ? (unsigned int)((G.ncells + 255) / 256) : G.tiles;
+        D.tiles = ew_blocks ? (unsigned int)((G.ncells + 255) / 256) : stage_tiles(G.n, b->dim, b->shape);
         D.first_block = first;
         D.nsamples = G.nsamples;
         first += D.tiles * (unsigned int)G.nsamples;
@@ -273,6 +273,72 @@ StatsParams stats_params(const avh_batch* b, unsigned int* nblocks) {
     return S;
 }
 
+TuneKey batch_tune_key(const avh_batch* b) {
+    long long total = 0;
+    for (int i = 0; i < b->ngroups; ++i) total += b->g[i].nsamples;
+    return TuneKey(1, b->cfg.equation * 16 + b->cfg.reconstruction, b->cfg.flux, b->dim, b->g[0].n[0],
+                   b->g[0].n[1], total);
+}
+
+// Times every tile shape with flux-only launches over all samples (they write
+// the L0 buffers and add to the per-sample wave-speed maxima; the control
+// blocks are restored afterwards) and keeps the fastest.
+avh_status batch_autotune(avh_batch* b, int* best, double* ms, int* nshapes) {
+    const int ns = num_shapes(b->dim);
+    AVH_HIP(hipStreamSynchronize(b->stream));
+    std::vector<std::vector<Control>> saved(b->ngroups);
+    for (int i = 0; i < b->ngroups; ++i) {
+        saved[i].resize(b->g[i].nsamples);
+        AVH_HIP(hipMemcpy(saved[i].data(), b->g[i].ctrl, sizeof(Control) * b->g[i].nsamples,
+                          hipMemcpyDeviceToHost));
+    }
+    const int keep = b->shape;
+    int bst = 0;
+    double tmin = 0;
+    avh_status r = AVH_OK;
+    for (int k = 0; k < ns && r == AVH_OK; ++k) {
+        double t = 0;
+        if (ns > 1) {
+            b->shape = k;
+            unsigned int nb = 0;
+            BatchParams P = batch_params(b, MODE_FLUX_ONLY, false, &nb);
+            for (int i = 0; i < b->ngroups; ++i) {
+                P.g[i].in = b->g[i].A;
+                P.g[i].u0 = b->g[i].A;
+                P.g[i].out = b->g[i].B;
+            }
+            r = time_launches(b->stream, 3, [&]() { return b->stage(P, nb, k, b->stream); }, &t);
+        }
+        if (ms) ms[k] = t;
+        if (k == 0 || t < tmin) {
+            tmin = t;
+            bst = k;
+        }
+    }
+    b->shape = keep;
+    AVH_HIP(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < b->ngroups; ++i)
+        AVH_HIP(hipMemcpy(b->g[i].ctrl, saved[i].data(), sizeof(Control) * b->g[i].nsamples,
+                          hipMemcpyHostToDevice));
+    if (r) return r;
+    if (b->shape != bst) b->graph.reset();
+    b->shape = bst;
+    tune_store(batch_tune_key(b), bst);
+    if (best) *best = bst;
+    if (nshapes) *nshapes = ns;
+    return AVH_OK;
+}
+
+avh_status batch_ensure_shape(avh_batch* b) {
+    if (b->shape >= 0) return AVH_OK;
+    int s;
+    if (tune_lookup(batch_tune_key(b), &s)) {
+        b->shape = s;
+        return AVH_OK;
+    }
+    return batch_autotune(b, nullptr, nullptr, nullptr);
+}
+
 // Launches per batch step: the stages (plus forward euler's / rk2's
 // element-wise pass), the end-of-step kernel and, with statistics, the
 // statistics kernel.
@@ -296,7 +362,7 @@ avh_status enqueue_batch_step(avh_batch* b, EventMarks* tm = nullptr) {
     enum { A = 0, B = 1, C = 2 };
     BatchParams P = batch_params(b, MODE_L, false, &nb);
     set_ptrs(P, A, A, B);
-    AVH_HIP(b->stage(P, nb, b->stream));
+    AVH_HIP(b->stage(P, nb, b->shape, b->stream));
     if ((r = mark(tm, b->stream))) return r;
     if (b->cfg.integrator == AVH_INT_FORWARDEULER) {
         BatchParams E = batch_params(b, MODE_L, true, &nbe);
@@ -312,16 +378,16 @@ avh_status enqueue_batch_step(avh_batch* b, EventMarks* tm = nullptr) {
         if ((r = mark(tm, b->stream))) return r;
         P = batch_params(b, MODE_RK2_2, false, &nb);
         set_ptrs(P, C, A, A);
-        AVH_HIP(b->stage(P, nb, b->stream));
+        AVH_HIP(b->stage(P, nb, b->shape, b->stream));
         if ((r = mark(tm, b->stream))) return r;
     } else {
         P = batch_params(b, MODE_RK3_2, false, &nb);
         set_ptrs(P, B, A, C);
-        AVH_HIP(b->stage(P, nb, b->stream));
+        AVH_HIP(b->stage(P, nb, b->shape, b->stream));
         if ((r = mark(tm, b->stream))) return r;
         P = batch_params(b, MODE_RK3_3, false, &nb);
         set_ptrs(P, C, A, A);
-        AVH_HIP(b->stage(P, nb, b->stream));
+        AVH_HIP(b->stage(P, nb, b->shape, b->stream));
         if ((r = mark(tm, b->stream))) return r;
     }
     unsigned int nbs = 0;
@@ -410,7 +476,6 @@ avh_status avh_batch_create(avh_ctx* ctx, const avh_config* cfg, int ngroups, co
         for (int d = 0; d < 3; ++d) G.n[d] = groups[i].n[d];
         G.nsamples = groups[i].nsamples;
         G.ncells = (long long)G.n[0] * G.n[1] * G.n[2];
-        G.tiles = stage_tiles(G.n, dim);
         const size_t state = (size_t)nv * G.ncells * G.nsamples * sizeof(double);
         const size_t stat = (size_t)std::max(1, b->nsaves) * nv * G.ncells * sizeof(double);
         bool ok = hipMalloc(&G.A, state) == hipSuccess && hipMalloc(&G.B, state) == hipSuccess &&
@@ -549,6 +614,8 @@ avh_status avh_batch_advance(avh_batch* b, int64_t max_steps, int chunk, int64_t
         AVH_HIP(hipGetLastError());
         b->initial_done = 1;
     }
+    avh_status rs = batch_ensure_shape(b);
+    if (rs) return rs;
     int64_t done = 0, left = 0;
     while (done < max_steps) {
         const int64_t k = std::min<int64_t>(chunk, max_steps - done);
@@ -579,13 +646,30 @@ avh_status avh_batch_set_graph_steps(avh_batch* b, int steps) {
     return AVH_OK;
 }
 
+avh_status avh_batch_set_tile_shape(avh_batch* b, int shape) {
+    if (!b) return fail(AVH_EINVAL, "null argument");
+    if (shape < -1 || shape >= num_shapes(b->dim)) return fail(AVH_EINVAL, "no such tile shape");
+    AVH_HIP(hipSetDevice(b->ctx->device));
+    AVH_HIP(hipStreamSynchronize(b->stream));
+    b->graph.reset();
+    b->shape = shape;
+    return AVH_OK;
+}
+
+avh_status avh_batch_autotune(avh_batch* b, int* best_shape, double* ms_per_shape, int* nshapes) {
+    if (!b) return fail(AVH_EINVAL, "null argument");
+    AVH_HIP(hipSetDevice(b->ctx->device));
+    return batch_autotune(b, best_shape, ms_per_shape, nshapes);
+}
+
 avh_status avh_batch_time_step_kernels(avh_batch* b, int64_t steps, double* ms, int* kernels) {
     if (!b || !ms || !kernels || steps < 1 || steps > 100000) return fail(AVH_EINVAL, "bad argument");
     AVH_HIP(hipSetDevice(b->ctx->device));
     const int nk = batch_launches_per_step(b);
-    EventMarks t;
-    avh_status r = t.create((size_t)steps * (nk + 1));
+    avh_status r = batch_ensure_shape(b);
     if (r) return r;
+    EventMarks t;
+    if ((r = t.create((size_t)steps * (nk + 1)))) return r;
     for (int64_t k = 0; k < steps; ++k)
         if ((r = enqueue_batch_step(b, &t))) return r;
     if ((r = t.average(steps, nk, ms))) return r;

@@ -119,6 +119,18 @@ class BatchRunner:
                                                  ctypes.byref(left)))
         return done.value, left.value
 
+    def set_tile_shape(self, shape):
+        """Tile shape of the stage kernel (-1: autotune on first use)."""
+        _abi.check(_abi.load().avh_batch_set_tile_shape(self._h, int(shape)))
+
+    def autotune(self):
+        """Time every tile shape now; returns (best shape, [ms per shape])."""
+        best = ctypes.c_int()
+        n = ctypes.c_int()
+        ms = (ctypes.c_double * 8)()
+        _abi.check(_abi.load().avh_batch_autotune(self._h, ctypes.byref(best), ms, ctypes.byref(n)))
+        return best.value, [ms[i] for i in range(n.value)]
+
     def set_graph_steps(self, steps):
         """Steps per captured hipGraph (0: ordinary launches)."""
         _abi.check(_abi.load().avh_batch_set_graph_steps(self._h, int(steps)))

@@ -19,6 +19,7 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0
+SHAPE_NAMES = {1: ["256", "64"], 2: ["32x16", "32x8", "64x4"], 3: ["32x8 columns x 32 planes"]}
 
 WORKLOADS = {
     # the bench line
@@ -100,6 +101,8 @@ def run_uq(args):
         br.set_kelvin_helmholtz(g, params[k:k + m])
         k += m
     cells = sum(n[0] * n[1] * m for n, m in UQ_GROUPS)
+    best, tune_ms = br.autotune()
+    tune = {"chosen": SHAPE_NAMES[2][best], "ms_per_shape": dict(zip(SHAPE_NAMES[2], tune_ms))}
     br.run(max_steps=args.warmup, chunk=max(1, args.warmup))
     print("bench: warmup done", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -159,6 +162,7 @@ def run_uq(args):
             "stage_ms": stage_ms,
             "end_of_step_kernel_ms": launch_ms[3],
             "statistics_kernel_ms": launch_ms[4] if len(launch_ms) > 4 else None,
+            "tile_autotune": tune,
         },
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
@@ -211,6 +215,9 @@ def main():
     print("bench: building %s state" % args.workload, file=sys.stderr, flush=True)
     sim = alsvinn_amd.Simulator(wl["equation"], "weno2", "hll3", "rungekutta3", "periodic", wl["n"], cfl=wl["cfl"])
     sim.set_interior(initial_state(wl))
+    best, tune_ms = sim.autotune()
+    dim = 3 if wl["n"][2] > 1 else (2 if wl["n"][1] > 1 else 1)
+    tune = {"chosen": SHAPE_NAMES[dim][best], "ms_per_shape": dict(zip(SHAPE_NAMES[dim], tune_ms))}
 
     sim.enqueue_steps(args.warmup)
     sim.synchronize()
@@ -285,6 +292,7 @@ def main():
             "algorithmic_bytes_per_launch": avg_bytes,
             "stage_ms": stage_ms,
             "end_of_step_kernel_ms": finish_ms,
+            "tile_autotune": tune,
         },
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,

@@ -156,6 +156,17 @@ avh_status avh_synchronize(avh_solver* solver);
  * once and replayed; results are identical either way.  The reference
  * launches each kernel separately (alsfvm/src/simulator/Simulator.cpp:191-214). */
 avh_status avh_set_graph_steps(avh_solver* solver, int steps);
+/* Tile shape of the fused stage kernel: -1 (default) lets the autotuner
+ * pick on first use (cached per equation, reconstruction, flux and grid for
+ * the process), 0..nshapes-1 fixes one (1D: 256 or 64 cells; 2D: 32x16,
+ * 32x8 or 64x4; 3D: one shape).  Results are identical for every shape.
+ * No reference counterpart: the reference's CUDA kernels use fixed block
+ * sizes (alsfvm/src/numflux/NumericalFluxCUDA.cu:330-414). */
+avh_status avh_set_tile_shape(avh_solver* solver, int shape);
+/* Time every tile shape on the solver's current state now (flux-only
+ * launches; the solution and step state are unchanged) and select the
+ * fastest.  ms_per_shape (may be NULL) receives *nshapes <= 4 timings. */
+avh_status avh_autotune(avh_solver* solver, int* best_shape, double* ms_per_shape, int* nshapes);
 
 /* Run `steps` full time steps with a HIP event pair around every kernel
  * launch on the solver's stream (the hipEvent analogue of the reference's
@@ -219,6 +230,10 @@ avh_status avh_batch_get_progress(avh_batch* batch, int group, int sample, avh_p
 /* Steps per captured hipGraph in avh_batch_advance (default 8; 0 or 1:
  * ordinary launches). */
 avh_status avh_batch_set_graph_steps(avh_batch* batch, int steps);
+/* Tile shape and autotuner of the batched stage kernel, as
+ * avh_set_tile_shape / avh_autotune. */
+avh_status avh_batch_set_tile_shape(avh_batch* batch, int shape);
+avh_status avh_batch_autotune(avh_batch* batch, int* best_shape, double* ms_per_shape, int* nshapes);
 /* Advance every sample `steps` full steps with a HIP event after every
  * launch; ms_per_launch[k] = average duration of the k-th launch of a step
  * (k < *kernels_per_step <= 6: stages, element-wise pass, end-of-step
