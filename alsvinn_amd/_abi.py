@@ -46,12 +46,25 @@ SYMBOLS = [
     "avh_batch_set_kh2d",
     "avh_batch_advance",
     "avh_batch_get_progress",
+    "avh_batch_set_end_time",
+    "avh_batch_kinetic_energy",
+    "avh_batch_structure_cube",
     "avh_batch_set_graph_steps",
     "avh_batch_set_tile_shape",
     "avh_batch_autotune",
     "avh_batch_time_step_kernels",
     "avh_batch_get_statistics",
     "avh_uq_generate",
+    "avh_mlp_create",
+    "avh_mlp_destroy",
+    "avh_mlp_num_params",
+    "avh_mlp_set_normalization",
+    "avh_mlp_get_params",
+    "avh_mlp_set_params",
+    "avh_mlp_gradients",
+    "avh_mlp_adam_step",
+    "avh_mlp_train",
+    "avh_mlp_predict",
 ]
 
 AVH_OK, AVH_EINVAL, AVH_EHIP, AVH_ECONSTRAINT, AVH_ENOMEM = range(5)
@@ -69,6 +82,21 @@ class AvhConfig(ctypes.Structure):
         ("upper", ctypes.c_double * 3),
         ("gamma", ctypes.c_double),
         ("cfl", ctypes.c_double),
+    ]
+
+
+class AvhMlpConfig(ctypes.Structure):
+    _fields_ = [
+        ("n_in", ctypes.c_int),
+        ("width", ctypes.c_int),
+        ("hidden", ctypes.c_int),
+        ("n_out", ctypes.c_int),
+        ("members", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
     ]
 
 
@@ -133,6 +161,9 @@ def load():
         "avh_set_tile_shape": [vp, ctypes.c_int],
         "avh_autotune": [vp, P(ctypes.c_int), P(d), P(ctypes.c_int)],
         "avh_batch_set_tile_shape": [vp, ctypes.c_int],
+        "avh_batch_set_end_time": [vp, d],
+        "avh_batch_kinetic_energy": [vp, ctypes.c_int, ctypes.c_int, vp],
+        "avh_batch_structure_cube": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, d, vp],
         "avh_batch_autotune": [vp, P(ctypes.c_int), P(d), P(ctypes.c_int)],
         "avh_batch_set_graph_steps": [vp, ctypes.c_int],
         "avh_batch_time_step_kernels": [vp, i64, P(d), P(ctypes.c_int)],
@@ -149,6 +180,16 @@ def load():
         "avh_batch_get_progress": [vp, ctypes.c_int, ctypes.c_int, P(AvhProgress)],
         "avh_batch_get_statistics": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, P(i64)],
         "avh_uq_generate": [ctypes.c_int, P(ctypes.c_int), P(d), P(d), i64, i64, vp],
+        "avh_mlp_create": [vp, P(AvhMlpConfig), P(vp)],
+        "avh_mlp_destroy": [vp],
+        "avh_mlp_num_params": [vp, P(i64)],
+        "avh_mlp_set_normalization": [vp, vp, vp, vp, vp],
+        "avh_mlp_get_params": [vp, ctypes.c_int, vp],
+        "avh_mlp_set_params": [vp, ctypes.c_int, vp],
+        "avh_mlp_gradients": [vp, vp, vp, i64, vp, vp],
+        "avh_mlp_adam_step": [vp, ctypes.c_float],
+        "avh_mlp_train": [vp, vp, vp, i64, ctypes.c_int, ctypes.c_float, ctypes.c_float, vp],
+        "avh_mlp_predict": [vp, vp, i64, vp],
     }
     for name, args in protos.items():
         f = getattr(L, name)

@@ -19,6 +19,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 SOURCES = [
     "avh_capi.hip",
     "avh_uq.hip",
+    "avh_mlp.hip",
     "avh_stage_burgers.hip",
     "avh_stage_euler1.hip",
     "avh_stage_euler2.hip",

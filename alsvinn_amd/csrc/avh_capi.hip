@@ -314,15 +314,15 @@ avh_status avh_ctx_destroy(avh_ctx* ctx) {
 
 avh_status avh_solver_destroy(avh_solver* s) {
     if (!s) return AVH_OK;
-    hipSetDevice(s->ctx->device);
-    if (s->stream) hipStreamSynchronize(s->stream);
+    (void)hipSetDevice(s->ctx->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     s->graph.reset();
-    hipFree(s->A);
-    hipFree(s->B);
-    hipFree(s->C);
-    hipFree(s->staging);
-    hipFree(s->ctrl);
-    if (s->stream) hipStreamDestroy(s->stream);
+    (void)hipFree(s->A);
+    (void)hipFree(s->B);
+    (void)hipFree(s->C);
+    (void)hipFree(s->staging);
+    (void)hipFree(s->ctrl);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return AVH_OK;
 }
@@ -378,9 +378,11 @@ avh_status avh_solver_create(avh_ctx* ctx, const avh_config* cfg, avh_solver** o
         avh_solver_destroy(s);
         return fail(AVH_EHIP, "hipStreamCreate failed");
     }
-    hipMemset(s->A, 0, bytes);
-    hipMemset(s->B, 0, bytes);
-    hipMemset(s->C, 0, bytes);
+    if (hipMemset(s->A, 0, bytes) != hipSuccess || hipMemset(s->B, 0, bytes) != hipSuccess ||
+        hipMemset(s->C, 0, bytes) != hipSuccess) {
+        avh_solver_destroy(s);
+        return fail(AVH_EHIP, "device initialisation failed");
+    }
     // Grid::computeCellLengths: (top - origin)/N, 1 on single-cell axes
     // (alsfvm/src/grid/Grid.cpp:24-41)
     Control& h = s->h;

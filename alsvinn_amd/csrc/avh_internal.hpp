@@ -74,7 +74,7 @@ struct EventMarks {
     EventMarks(const EventMarks&) = delete;
     EventMarks& operator=(const EventMarks&) = delete;
     ~EventMarks() {
-        for (auto e : ev) hipEventDestroy(e);
+        for (auto e : ev) (void)hipEventDestroy(e);
     }
     avh_status create(size_t n) {
         ev.reserve(n);
@@ -118,8 +118,8 @@ struct StepGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void reset() {
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
         steps = 0;
@@ -141,14 +141,14 @@ avh_status enqueue_steps_graphed(StepGraph& G, int graph_steps, hipStream_t st, 
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(st, &g);
             if (r != AVH_OK) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 return r;
             }
             if (e != hipSuccess) return fail(AVH_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
             hipGraphExec_t x = nullptr;
             const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
             if (e2 != hipSuccess) {
-                hipGraphDestroy(g);
+                (void)hipGraphDestroy(g);
                 return fail(AVH_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
             }
             G.graph = g;
@@ -202,7 +202,7 @@ avh_status time_launches(hipStream_t st, int reps, Launch&& launch, double* ms) 
     hipEvent_t e0, e1;
     AVH_HIP(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
         return fail(AVH_EHIP, "hipEventCreate failed");
     }
     hipError_t e = launch();
@@ -212,8 +212,8 @@ avh_status time_launches(hipStream_t st, int reps, Launch&& launch, double* ms) 
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float f = 0;
     if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (e != hipSuccess) return fail(AVH_EHIP, std::string("autotune timing: ") + hipGetErrorString(e));
     *ms = f / reps;
     return AVH_OK;

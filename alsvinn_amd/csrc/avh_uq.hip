@@ -121,6 +121,134 @@ __global__ __launch_bounds__(256) void kh2d_fill_kernel(double* state, long long
 }
 
 
+// Kinetic energy of one velocity component per sample,
+//   E_c = (sum over cells of 0.5 * rho * u_c * u_c) * dV,  u_c = m_c / rho,
+// a functional of the conserved state modelled on the Moment functional
+// (alsfvm/src/functional/Moment.cpp:31-45).  One block per sample; each
+// thread sums a strided subset in cell order, then a fixed tree, so the
+// result does not depend on timing.
+__global__ __launch_bounds__(256) void kinetic_energy_kernel(const double* state, long long stride, long long ncells,
+                                                             int comp, double dv, double* out) {
+    __shared__ double red[256];
+    const double* q = state + (long long)blockIdx.x * stride;
+    double acc = 0.0;
+    for (long long i = threadIdx.x; i < ncells; i += 256) {
+        const double rho = q[i];
+        const double u = q[(long long)(1 + comp) * ncells + i] / rho;
+        acc = acc + 0.5 * rho * (u * u);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0] * dv;
+}
+
+template <int P>
+__device__ __forceinline__ double fast_power(double x) {
+    if constexpr (P == 1) {
+        return x;
+    } else {
+        return x * fast_power<P - 1>(x);
+    }
+}
+
+__device__ __forceinline__ int bc_value_index(int x, int n, int bc) {
+    // ValueAtBoundary: periodic wraps, neumann clamps to the interior
+    // (alsfvm/include/alsfvm/boundary/ValueAtBoundary.hpp)
+    if (bc == AVH_BC_PERIODIC) {
+        while (x < 0) x += n;
+        return x % n;
+    }
+    return x < 0 ? 0 : (x > n - 1 ? n - 1 : x);
+}
+
+// Structure function of one variable per sample (the structure_cube
+// statistic, alsfvm/include/alsfvm/functional/structure_common.hpp:36-154):
+// for every cell and lag h, |u - u'|^p summed over the cells u' on the
+// boundary of the square of half-width h around it (2 points in 1D, 8h in
+// 2D), divided by the number of cells.  Block = 256 consecutive cells of one
+// sample at one lag; partial sums in fixed order.
+template <int P>
+__global__ __launch_bounds__(256) void structure_partial_kernel(const double* state, long long stride, int var, int nx,
+                                                                int ny, int dims, int bc, double p, double* partial) {
+    __shared__ double red[256];
+    const long long ncells = (long long)nx * ny;
+    const int h = blockIdx.y + 1;
+    const double* u = state + (long long)blockIdx.z * stride + (long long)var * ncells;
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (c < ncells) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        const double uc = u[c];
+        auto term = [&](int x, int y) {
+            const double d = fabs(uc - u[(long long)bc_value_index(y, ny, bc) * nx + bc_value_index(x, nx, bc)]);
+            if constexpr (P > 0) {
+                acc = acc + fast_power<P>(d);
+            } else {
+                acc = acc + pow(d, p);
+            }
+        };
+        if (dims == 1) {
+            term(i - h, 0);
+            term(i + h, 0);
+        } else {
+            // d = 0: columns x = i -/+ h, y in [j-h+1, j+h-1]
+            for (int side = 0; side < 2; ++side) {
+                const int x = side == 0 ? i - h : i + h;
+                for (int y = j - h + 1; y < j + h; ++y) term(x, y);
+            }
+            // d = 1: rows y = j -/+ h, x in [i-h, i+h]
+            for (int side = 0; side < 2; ++side) {
+                const int y = side == 0 ? j - h : j + h;
+                for (int x = i - h; x < i + h + 1; ++x) term(x, y);
+            }
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        partial[((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// out[s][0] = 0, out[s][h] = (sum of the partials of (s, h) in block order) / cells
+__global__ __launch_bounds__(256) void structure_finish_kernel(const double* partial, int nchunks, int nh, int nsamples,
+                                                               double ncells, double* out) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nsamples * nh) return;
+    const int s = t / nh, h = t % nh;
+    if (h == 0) {
+        out[t] = 0.0;
+        return;
+    }
+    const double* q = partial + ((long long)s * (nh - 1) + (h - 1)) * nchunks;
+    double acc = 0.0;
+    for (int k = 0; k < nchunks; ++k) acc = acc + q[k];
+    out[t] = acc / ncells;
+}
+
+// End time of every sample: samples with t < t_end (and no constraint
+// failure) run again.
+__global__ __launch_bounds__(256) void set_end_time_kernel(Control* ctrl, int nsamples, double t_end, int* active_count) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < nsamples; s += 256) {
+        Control* c = ctrl + s;
+        c->t_end = t_end;
+        c->active = (!(c->t >= t_end) && !c->violated) ? 1 : 0;
+        if (c->active) atomicAdd(&cnt, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *active_count = cnt;
+}
+
 }  // namespace avh
 
 namespace {
@@ -194,25 +322,25 @@ namespace {
 
 void free_batch(avh_batch* b) {
     if (!b) return;
-    hipSetDevice(b->ctx->device);
-    if (b->stream) hipStreamSynchronize(b->stream);
+    (void)hipSetDevice(b->ctx->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     b->graph.reset();
     for (int i = 0; i < b->ngroups; ++i) {
         GroupHost& G = b->g[i];
-        hipFree(G.A);
-        hipFree(G.B);
-        hipFree(G.C);
-        hipFree(G.ctrl);
-        hipFree(G.mean_c);
-        hipFree(G.m2_c);
-        hipFree(G.mean_p);
-        hipFree(G.m2_p);
-        hipFree(G.list);
-        hipFree(G.list_len);
-        hipFree(G.counts);
-        hipFree(G.active_count);
+        (void)hipFree(G.A);
+        (void)hipFree(G.B);
+        (void)hipFree(G.C);
+        (void)hipFree(G.ctrl);
+        (void)hipFree(G.mean_c);
+        (void)hipFree(G.m2_c);
+        (void)hipFree(G.mean_p);
+        (void)hipFree(G.m2_p);
+        (void)hipFree(G.list);
+        (void)hipFree(G.list_len);
+        (void)hipFree(G.counts);
+        (void)hipFree(G.active_count);
     }
-    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
@@ -596,7 +724,7 @@ avh_status avh_batch_set_kh2d(avh_batch* b, int group, const double* host_interf
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-    hipFree(d);
+    (void)hipFree(d);
     if (e != hipSuccess) return fail(AVH_EHIP, std::string("KH fill: ") + hipGetErrorString(e));
     return AVH_OK;
 }
@@ -643,6 +771,88 @@ avh_status avh_batch_set_graph_steps(avh_batch* b, int steps) {
     AVH_HIP(hipStreamSynchronize(b->stream));
     b->graph.reset();
     b->graph_steps = steps;
+    return AVH_OK;
+}
+
+avh_status avh_batch_set_end_time(avh_batch* b, double t_end) {
+    if (!b) return fail(AVH_EINVAL, "null argument");
+    AVH_HIP(hipSetDevice(b->ctx->device));
+    for (int i = 0; i < b->ngroups; ++i) {
+        hipLaunchKernelGGL(set_end_time_kernel, dim3(1), dim3(256), 0, b->stream, b->g[i].ctrl, b->g[i].nsamples, t_end,
+                           b->g[i].active_count);
+        AVH_HIP(hipGetLastError());
+    }
+    b->proto.t_end = t_end;
+    AVH_HIP(hipStreamSynchronize(b->stream));
+    return AVH_OK;
+}
+
+avh_status avh_batch_kinetic_energy(avh_batch* b, int group, int component, double* out) {
+    avh_status r = check_gs(b, group, 0);
+    if (r) return r;
+    if (!out) return fail(AVH_EINVAL, "null argument");
+    if (b->cfg.equation == AVH_EQ_BURGERS || component < 0 || component >= b->nv - 2)
+        return fail(AVH_EINVAL, "kinetic energy needs an euler equation and a velocity component < dimensions");
+    AVH_HIP(hipSetDevice(b->ctx->device));
+    const GroupHost& G = b->g[group];
+    double dv = 1.0;
+    for (int d = 0; d < 3; ++d)
+        if (G.n[d] > 1) dv *= (b->cfg.upper[d] - b->cfg.lower[d]) / double(G.n[d]);
+    double* d = nullptr;
+    AVH_HIP(hipMalloc(&d, sizeof(double) * G.nsamples));
+    hipLaunchKernelGGL(kinetic_energy_kernel, dim3(G.nsamples), dim3(256), 0, b->stream, G.A,
+                       (long long)b->nv * G.ncells, G.ncells, component, dv, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, sizeof(double) * G.nsamples, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(AVH_EHIP, std::string("kinetic energy: ") + hipGetErrorString(e));
+    return AVH_OK;
+}
+
+avh_status avh_batch_structure_cube(avh_batch* b, int group, int var, int number_of_h, double p, double* out) {
+    avh_status r = check_gs(b, group, 0);
+    if (r) return r;
+    if (!out) return fail(AVH_EINVAL, "null argument");
+    if (var < 0 || var >= b->nv) return fail(AVH_EINVAL, "no such variable");
+    if (b->dim > 2) return fail(AVH_EINVAL, "structure functions of batches: 1D and 2D grids");
+    const GroupHost& G = b->g[group];
+    if (number_of_h < 2 || number_of_h > 65535) return fail(AVH_EINVAL, "number_of_h must be in [2, 65535]");
+    if (!(p > 0)) return fail(AVH_EINVAL, "p must be positive");
+    if (G.nsamples > 65535) return fail(AVH_EINVAL, "structure functions: at most 65535 samples per group");
+    AVH_HIP(hipSetDevice(b->ctx->device));
+    const unsigned int nchunks = (unsigned int)((G.ncells + 255) / 256);
+    const size_t npart = (size_t)nchunks * (number_of_h - 1) * G.nsamples;
+    double *part = nullptr, *d = nullptr;
+    AVH_HIP(hipMalloc(&part, sizeof(double) * npart));
+    if (hipMalloc(&d, sizeof(double) * G.nsamples * number_of_h) != hipSuccess) {
+        (void)hipFree(part);
+        return fail(AVH_ENOMEM, "device allocation failed");
+    }
+    const dim3 grid(nchunks, number_of_h - 1, G.nsamples);
+    const long long stride = (long long)b->nv * G.ncells;
+    const int pi = p == 1.0 ? 1 : p == 2.0 ? 2 : p == 3.0 ? 3 : p == 4.0 ? 4 : p == 5.0 ? 5 : 0;
+    switch (pi) {
+    case 1: hipLaunchKernelGGL(structure_partial_kernel<1>, grid, dim3(256), 0, b->stream, G.A, stride, var, G.n[0], G.n[1], b->dim, b->cfg.boundary, p, part); break;
+    case 2: hipLaunchKernelGGL(structure_partial_kernel<2>, grid, dim3(256), 0, b->stream, G.A, stride, var, G.n[0], G.n[1], b->dim, b->cfg.boundary, p, part); break;
+    case 3: hipLaunchKernelGGL(structure_partial_kernel<3>, grid, dim3(256), 0, b->stream, G.A, stride, var, G.n[0], G.n[1], b->dim, b->cfg.boundary, p, part); break;
+    case 4: hipLaunchKernelGGL(structure_partial_kernel<4>, grid, dim3(256), 0, b->stream, G.A, stride, var, G.n[0], G.n[1], b->dim, b->cfg.boundary, p, part); break;
+    case 5: hipLaunchKernelGGL(structure_partial_kernel<5>, grid, dim3(256), 0, b->stream, G.A, stride, var, G.n[0], G.n[1], b->dim, b->cfg.boundary, p, part); break;
+    default: hipLaunchKernelGGL(structure_partial_kernel<0>, grid, dim3(256), 0, b->stream, G.A, stride, var, G.n[0], G.n[1], b->dim, b->cfg.boundary, p, part); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        const int nt = G.nsamples * number_of_h;
+        hipLaunchKernelGGL(structure_finish_kernel, dim3((nt + 255) / 256), dim3(256), 0, b->stream, part, (int)nchunks,
+                           number_of_h, G.nsamples, (double)G.ncells, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d, sizeof(double) * G.nsamples * number_of_h, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(part);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(AVH_EHIP, std::string("structure function: ") + hipGetErrorString(e));
     return AVH_OK;
 }
 

@@ -119,6 +119,24 @@ class BatchRunner:
                                                  ctypes.byref(left)))
         return done.value, left.value
 
+    def set_end_time(self, t_end):
+        """Every sample runs to t_end (again active if t < t_end)."""
+        self.end_time = float(t_end)
+        _abi.check(_abi.load().avh_batch_set_end_time(self._h, float(t_end)))
+
+    def kinetic_energy(self, group, component=1):
+        """Per sample: sum of 0.5 rho u_c^2 dV at the current state."""
+        out = np.empty(self.groups[group][1])
+        _abi.check(_abi.load().avh_batch_kinetic_energy(self._h, int(group), int(component), out.ctypes.data))
+        return out
+
+    def structure_cube(self, group, var=0, number_of_h=63, p=2.0):
+        """Per sample structure function S_p(h), h = 0..number_of_h-1 (S_p(0) = 0)."""
+        out = np.empty((self.groups[group][1], int(number_of_h)))
+        _abi.check(_abi.load().avh_batch_structure_cube(self._h, int(group), int(var), int(number_of_h), float(p),
+                                                        out.ctypes.data))
+        return out
+
     def set_tile_shape(self, shape):
         """Tile shape of the stage kernel (-1: autotune on first use)."""
         _abi.check(_abi.load().avh_batch_set_tile_shape(self._h, int(shape)))

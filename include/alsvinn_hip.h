@@ -227,6 +227,23 @@ avh_status avh_batch_set_kh2d(avh_batch* batch, int group, const double* host_in
 avh_status avh_batch_advance(avh_batch* batch, int64_t max_steps, int chunk, int64_t* steps_done,
                              int64_t* active_left);
 avh_status avh_batch_get_progress(avh_batch* batch, int group, int sample, avh_progress* progress);
+/* Set every sample's end time (Simulator::setEndTime analogue): samples with
+ * time < t_end and no constraint failure become active again.  With a
+ * fixed-interval adjuster whose save times include t_end, every sample stops
+ * exactly at t_end, so observables can be read there. */
+avh_status avh_batch_set_end_time(avh_batch* batch, double t_end);
+/* out[s] = (sum over cells of 0.5 * rho * u_c^2) * dV for each sample s of the
+ * group at its current state, u_c = m_c / rho, c < dimensions (euler only); a
+ * functional modelled on alsfvm/src/functional/Moment.cpp:31-45. */
+avh_status avh_batch_kinetic_energy(avh_batch* batch, int group, int component, double* out);
+/* Structure function of variable `var` per sample at its current state, the
+ * structure_cube statistic (alsuq/src/stats/StructureCube.cpp:37-76,
+ * alsfvm/include/alsfvm/functional/structure_common.hpp:36-154): out[s][h] =
+ * sum over cells and over the cells on the square ring at distance h of
+ * |u - u'|^p, divided by the number of cells, h < number_of_h (out[s][0] = 0);
+ * ring cells through the grid's boundary condition.  1D and 2D grids. */
+avh_status avh_batch_structure_cube(avh_batch* batch, int group, int var, int number_of_h, double p,
+                                    double* out);
 /* Steps per captured hipGraph in avh_batch_advance (default 8; 0 or 1:
  * ordinary launches). */
 avh_status avh_batch_set_graph_steps(avh_batch* batch, int steps);
@@ -253,6 +270,50 @@ avh_status avh_batch_get_statistics(avh_batch* batch, int group, int save, int p
  * given order, each dims[p] long, scaled to [lower[p], upper[p]). */
 avh_status avh_uq_generate(int nparams, const int* dims, const double* lower, const double* upper,
                            int64_t sample_start, int64_t nsamples, double* out);
+
+/* ---------------------------------------------------------------------------
+ * Surrogate network: an ensemble of multilayer perceptrons n_in -> width
+ * (GELU) x hidden -> n_out, bf16 weights and activations on the matrix
+ * cores with fp32 accumulation, fp32 master weights trained by Adam on the
+ * normalised mean-squared error.  No reference counterpart: the reference
+ * has no learned model; this maps the sample parameters of its Monte Carlo
+ * runs (alsuq/src/run/Runner.cpp:39-90) to their observables.
+ * ------------------------------------------------------------------------- */
+typedef struct avh_mlp avh_mlp;
+
+typedef struct avh_mlp_config {
+    int n_in;      /* inputs (sample parameters) */
+    int width;     /* hidden width, multiple of 16 */
+    int hidden;    /* hidden layers, 1..7 */
+    int n_out;     /* outputs (observables) */
+    int members;   /* ensemble members, 1..64 */
+    int batch;     /* minibatch rows, multiple of 16 */
+    float beta1, beta2, eps;  /* Adam */
+    uint64_t seed; /* initial weights and shuffles */
+} avh_mlp_config;
+
+avh_status avh_mlp_create(avh_ctx* ctx, const avh_mlp_config* cfg, avh_mlp** out);
+avh_status avh_mlp_destroy(avh_mlp* mlp);
+/* parameters per member: sum over layers of in * out + out */
+avh_status avh_mlp_num_params(avh_mlp* mlp, int64_t* count);
+/* inputs and targets are standardised as (x - mean) / std inside */
+avh_status avh_mlp_set_normalization(avh_mlp* mlp, const float* in_mean, const float* in_std,
+                                     const float* out_mean, const float* out_std);
+/* fp32 parameters of one member: per layer W[out][in] row-major, then b[out] */
+avh_status avh_mlp_get_params(avh_mlp* mlp, int member, float* params);
+avh_status avh_mlp_set_params(avh_mlp* mlp, int member, const float* params);
+/* loss and gradients of every member on the same `batch` rows, no update:
+ * X[rows][n_in], Y[rows][n_out], grads[member][num_params], loss[member] */
+avh_status avh_mlp_gradients(avh_mlp* mlp, const float* X, const float* Y, int64_t rows, float* grads,
+                             float* loss);
+/* one Adam step of every member from the last gradients */
+avh_status avh_mlp_adam_step(avh_mlp* mlp, float lr);
+/* minibatch Adam over `epochs` passes, cosine learning-rate decay from lr to
+ * lr * final_lr_fraction; epoch_loss[epoch][member] (may be NULL) */
+avh_status avh_mlp_train(avh_mlp* mlp, const float* X, const float* Y, int64_t rows, int epochs, float lr,
+                         float final_lr_fraction, float* epoch_loss);
+/* out[member][row][n_out] in data units */
+avh_status avh_mlp_predict(avh_mlp* mlp, const float* X, int64_t rows, float* out);
 
 #ifdef __cplusplus
 }

@@ -1214,4 +1214,72 @@ void oracle_uniform01(uint64_t seed, uint64_t skip, uint64_t n, double* out) {
     for (uint64_t i = 0; i < n; ++i) out[i] = dist(g);
 }
 
+/* Kinetic energy of velocity component `comp` of one interior SoA state
+ * (nv variables of ncells each): (sum in cell order of 0.5*rho*u*u) * dv,
+ * u = m_comp / rho.  A new functional; the Moment functional
+ * (alsfvm/src/functional/Moment.cpp:31-45) is its model. */
+double oracle_kinetic_energy(const double* q, long ncells, int comp, double dv) {
+    double acc = 0;
+    for (long i = 0; i < ncells; ++i) {
+        const double rho = q[i];
+        const double u = q[(long)(1 + comp) * ncells + i] / rho;
+        acc += 0.5 * rho * (u * u);
+    }
+    return acc * dv;
+}
+
+static int value_at_boundary(int x, int n, int bc) {
+    /* ValueAtBoundary<PERIODIC/NEUMANN> (alsfvm/include/alsfvm/boundary/ValueAtBoundary.hpp) */
+    if (bc == 0) {
+        while (x < 0) x += n;
+        return x % n;
+    }
+    return std::max(0, std::min(x, n - 1));
+}
+
+static double structure_power(double x, double p) {
+    /* FastPower<k> for p = 1..5 (x * (x * ...)), PowPower otherwise
+     * (alsfvm/include/alsfvm/functional/structure_common.hpp:156-180) */
+    if (p == 1.0) return x;
+    if (p == 2.0) return x * x;
+    if (p == 3.0) return x * (x * x);
+    if (p == 4.0) return x * (x * (x * x));
+    if (p == 5.0) return x * (x * (x * (x * x)));
+    return std::pow(x, p);
+}
+
+/* computeStructureCubeCPU for one variable of an interior field u[ny][nx]
+ * (dims 1 or 2): for k, j, i in order, for h = 1..number_of_h-1, the
+ * forEachPointInComputeStructureCube ring in its loop order, each term
+ * divided by the cell count as it is added
+ * (alsfvm/include/alsfvm/functional/structure_common.hpp:36-154). */
+void oracle_structure_cube(const double* u, int nx, int ny, int dims, int bc, int number_of_h, double p,
+                           double* out) {
+    const int nz = 1;
+    for (int h = 0; h < number_of_h; ++h) out[h] = 0;
+    const double ncell = double(nx * ny * nz);
+    for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+            const double uc = u[(long)j * nx + i];
+            for (int h = 1; h < number_of_h; ++h) {
+                for (int d = 0; d < dims; ++d) {
+                    for (int side = 0; side < 2; ++side) {
+                        const bool yDir = d == 1, xDir = d == 0;
+                        const int yStart = yDir ? (side == 0 ? j - h : j + h) : (dims == 2 ? j - h + 1 : 0);
+                        const int yEnd = yDir ? yStart + 1 : (dims == 2 ? j + h : 1);
+                        const int xStart = xDir ? (side == 0 ? i - h : i + h) : i - h;
+                        const int xEnd = xDir ? xStart + 1 : i + h + 1;
+                        for (int y = yStart; y < yEnd; ++y)
+                            for (int x = xStart; x < xEnd; ++x) {
+                                const double v = u[(long)value_at_boundary(y, ny, bc) * nx +
+                                                   value_at_boundary(x, nx, bc)];
+                                out[h] += structure_power(std::fabs(uc - v), p) / ncell;
+                            }
+                    }
+                }
+            }
+        }
+    }
+}
+
 } /* extern "C" */

@@ -113,6 +113,10 @@ def lib():
         L.oracle_integrate_ode.restype = d
         L.oracle_mt19937_64.argtypes = [ctypes.c_uint64] * 3 + [P(ctypes.c_uint64)]
         L.oracle_uniform01.argtypes = [ctypes.c_uint64] * 3 + [P(d)]
+        L.oracle_kinetic_energy.argtypes = [P(d), ctypes.c_long, ctypes.c_int, d]
+        L.oracle_kinetic_energy.restype = d
+        L.oracle_structure_cube.argtypes = [P(d), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, d, P(d)]
         _lib = L
     return _lib
 
@@ -331,3 +335,20 @@ def flux_all(nsd, flux, direction, left_all, right_all, gamma=1.4):
 
 def integrate_ode(integrator, h, steps, u0=1.0):
     return lib().oracle_integrate_ode(INTEGRATOR[integrator], float(h), int(steps), float(u0))
+
+
+def kinetic_energy(interior, component, dv):
+    """sum of 0.5*rho*u_c^2 over the interior SoA state, times dv."""
+    q = np.ascontiguousarray(interior, dtype=np.float64)
+    ncells = q[0].size
+    return lib().oracle_kinetic_energy(_dp(q), ncells, int(component), float(dv))
+
+
+def structure_cube(field, boundary, number_of_h, p=2.0):
+    """Structure function of one 1D (nx,) or 2D (ny, nx) field, reference loop order."""
+    u = np.ascontiguousarray(field, dtype=np.float64)
+    dims = 1 if u.ndim == 1 or u.shape[0] == 1 else 2
+    ny, nx = (1, u.shape[-1]) if dims == 1 else u.shape[-2:]
+    out = np.zeros(int(number_of_h))
+    lib().oracle_structure_cube(_dp(u), int(nx), int(ny), dims, BC[boundary], int(number_of_h), float(p), _dp(out))
+    return out
