@@ -193,3 +193,182 @@ def surrogate_statistics(model, X_query, solver=None, threshold=0.05, X_solved=N
         res["cv_mean"] = m1
         res["cv_var"] = m2 - m1 * m1
     return res
+
+
+def control_variate_efficiency(model, X_solved, Y_solved):
+    """Per observable var(f) / var(f - g) on solver samples the network did
+    not train on: the factor by which the control-variate estimator cuts the
+    number of solver samples for a given error (1 = no gain)."""
+    g, _ = model.predict(X_solved)
+    f = np.asarray(Y_solved, dtype=np.float64)
+    return f.var(0) / np.maximum((f - g).var(0), 1e-300)
+
+
+def _rel_err(est, ref):
+    return np.abs(est - ref) / np.maximum(np.abs(ref), 1e-300)
+
+
+def plain_mc_error(pool, ref_mean, ref_var, nsamples, reps=16, seed=0):
+    """Mean over observables of the relative error of plain Monte Carlo mean
+    and variance with `nsamples` samples, averaged over `reps` random subsets
+    of the solver pool."""
+    rng = np.random.default_rng(seed)
+    em, ev = [], []
+    for _ in range(reps):
+        sub = pool[rng.choice(pool.shape[0], nsamples, replace=False)]
+        em.append(_rel_err(sub.mean(0), ref_mean).mean())
+        ev.append(_rel_err(sub.var(0), ref_var).mean())
+    return float(np.mean(em)), float(np.mean(ev))
+
+
+def uq_experiment(n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), number_of_h=63, n_train=4096, n_holdout=512,
+                  n_ref=16384, n_query=16384, epochs=300, lr=1e-3, members=5, width=256, hidden=4, batch=256,
+                  threshold=0.05, chunk=2048, data=None, log=None, seed=0):
+    """Surrogate-assisted UQ of the Kelvin-Helmholtz observables against plain
+    Monte Carlo.  Sample ids: training [0, n_train), held out next, then the
+    plain-solver reference, then the surrogate's query samples.  `data` may
+    hold solver observables from an earlier call ({"train", "holdout", "ref"}
+    arrays plus "solver_seconds_per_sample") to skip those runs.  Returns a
+    dict of errors and wall times."""
+    log = log or (lambda *a: None)
+    data = dict(data or {})
+    n_out = len(times) + number_of_h - 1
+    ids = {"train": (0, n_train), "holdout": (n_train, n_holdout), "ref": (n_train + n_holdout, n_ref)}
+    spent = {}
+    for key, (start, count) in ids.items():
+        if key in data and data[key].shape == (count, n_out):
+            continue
+        t0 = time.perf_counter()
+        data[key] = kh_observables(kh_parameters(start, count), n=n, end_time=end_time, times=times,
+                                   number_of_h=number_of_h, chunk=chunk,
+                                   chunk_callback=lambda k, key=key: log("solver %s: %d samples" % (key, k)))
+        spent[key] = time.perf_counter() - t0
+    if spent:
+        data["solver_seconds_per_sample"] = sum(spent.values()) / sum(ids[k][1] for k in spent)
+    t_sample = float(data["solver_seconds_per_sample"])
+    Xtr = kh_parameters(*ids["train"])
+    Xho = kh_parameters(*ids["holdout"])
+    q0 = n_train + n_holdout + n_ref
+    Xq = kh_parameters(q0, n_query)
+    Ytr, Yho, Yref = data["train"], data["holdout"], data["ref"]
+    ref_mean, ref_var = Yref.mean(0), Yref.var(0)
+
+    model = EnsembleMLP(40, n_out, width=width, hidden=hidden, members=members, batch=batch, seed=seed)
+    model.fit_normalization(Xtr, Ytr)
+    t0 = time.perf_counter()
+    hist = model.train(Xtr, Ytr, epochs=epochs, lr=lr)
+    t_train = time.perf_counter() - t0
+    log("trained %d epochs in %.1f s, loss %.3g -> %.3g" % (epochs, t_train, hist[0].mean(), hist[-1].mean()))
+    g_ho, s_ho = model.predict(Xho)
+    holdout_nmse = (np.mean((g_ho - Yho) ** 2, axis=0) / np.maximum(Yho.var(0), 1e-300))
+
+    def solver(X):
+        start = time.perf_counter()
+        # fallback samples are fresh query samples: their parameters are given
+        out = kh_observables(X, n=n, end_time=end_time, times=times, number_of_h=number_of_h, chunk=chunk)
+        log("fallback solver: %d samples in %.1f s" % (X.shape[0], time.perf_counter() - start))
+        return out
+
+    st = surrogate_statistics(model, Xq, solver=solver, threshold=threshold, X_solved=Xho, Y_solved=Yho)
+    res = {
+        "n": n, "end_time": end_time, "n_train": n_train, "n_holdout": n_holdout, "n_ref": n_ref,
+        "n_query": n_query, "epochs": epochs, "solver_seconds_per_sample": t_sample, "train_seconds": t_train,
+        "final_loss": float(hist[-1].mean()), "holdout_nmse_mean": float(holdout_nmse.mean()),
+        "holdout_nmse_max": float(holdout_nmse.max()), "fallback_fraction": st["fallback_fraction"],
+        "predict_seconds": st["predict_seconds"], "fallback_solver_seconds": st["solver_seconds"],
+    }
+    # errors of each estimator against the plain-solver reference
+    for name, m, v in (("fallback", st["fallback_mean"], st["fallback_var"]), ("cv", st["cv_mean"], st["cv_var"])):
+        res[name + "_mean_err"] = float(_rel_err(m, ref_mean).mean())
+        res[name + "_var_err"] = float(_rel_err(v, ref_var).mean())
+    eff = control_variate_efficiency(model, Xho, Yho)
+    res["cv_efficiency_median"] = float(np.median(eff))
+    res["cv_efficiency_min"] = float(eff.min())
+    res["cv_efficiency_max"] = float(eff.max())
+    res["holdout_nmse"] = holdout_nmse.tolist()
+    res["cv_efficiency"] = eff.tolist()
+    budget = n_train + n_holdout
+    res["plain_mean_err_at_budget"], res["plain_var_err_at_budget"] = plain_mc_error(Yref, ref_mean, ref_var,
+                                                                                      min(budget, n_ref // 2))
+    # plain Monte Carlo samples for the surrogate's error (1/sqrt(N) scaling
+    # from the measured error at the budget), and its wall time
+    wall_sur = budget * t_sample + t_train + st["predict_seconds"] + st["solver_seconds"]
+    res["surrogate_wall_seconds"] = wall_sur
+    for name in ("fallback", "cv"):
+        e_m = max(res[name + "_mean_err"], 1e-300)
+        e_v = max(res[name + "_var_err"], 1e-300)
+        n_m = min(budget, n_ref // 2) * (res["plain_mean_err_at_budget"] / e_m) ** 2
+        n_v = min(budget, n_ref // 2) * (res["plain_var_err_at_budget"] / e_v) ** 2
+        n_eq = max(n_m, n_v)
+        res[name + "_plain_equivalent_samples"] = n_eq
+        res[name + "_speedup"] = n_eq * t_sample / wall_sur
+    return res, data
+
+
+def main(argv=None):
+    """Staged driver for the Kelvin-Helmholtz surrogate study (one GPU call
+    per stage, observables cached as .npz in --data):
+
+      --stage train  solver observables of the training and held-out samples
+      --stage ref    a part of the plain-solver reference (--ref-start, --ref-count)
+      --stage eval   train the ensemble, query, fall back, compare; prints JSON
+    """
+    import argparse
+    import glob
+    import json
+    import os
+
+    ap = argparse.ArgumentParser(description=main.__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--stage", choices=["train", "ref", "eval"], required=True)
+    ap.add_argument("--data", required=True, help="directory of the cached observables")
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--end-time", type=float, default=2.0)
+    ap.add_argument("--n-train", type=int, default=4096)
+    ap.add_argument("--n-holdout", type=int, default=512)
+    ap.add_argument("--n-ref", type=int, default=16384)
+    ap.add_argument("--ref-start", type=int, default=0)
+    ap.add_argument("--ref-count", type=int, default=4096)
+    ap.add_argument("--n-query", type=int, default=16384)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--chunk", type=int, default=2048)
+    args = ap.parse_args(argv)
+    os.makedirs(args.data, exist_ok=True)
+    t0 = time.perf_counter()
+
+    def log(*a):
+        print("[%.1f s]" % (time.perf_counter() - t0), *a, flush=True)
+
+    times = tuple(args.end_time * k / 4 for k in (1, 2, 3, 4))
+    kw = dict(n=args.n, end_time=args.end_time, times=times, number_of_h=63, chunk=args.chunk)
+    if args.stage == "train":
+        out = {}
+        for key, start, count in (("train", 0, args.n_train), ("holdout", args.n_train, args.n_holdout)):
+            ts = time.perf_counter()
+            out[key] = kh_observables(kh_parameters(start, count), chunk_callback=lambda k: log(key, k), **kw)
+            out[key + "_seconds"] = time.perf_counter() - ts
+        np.savez(os.path.join(args.data, "kh_train.npz"), **out)
+        log("saved", {k: np.shape(v) for k, v in out.items()})
+    elif args.stage == "ref":
+        start = args.n_train + args.n_holdout + args.ref_start
+        ts = time.perf_counter()
+        Y = kh_observables(kh_parameters(start, args.ref_count), chunk_callback=lambda k: log("ref", k), **kw)
+        np.savez(os.path.join(args.data, "kh_ref_%06d.npz" % args.ref_start), ref=Y,
+                 seconds=time.perf_counter() - ts)
+        log("saved ref part", args.ref_start, Y.shape)
+    else:
+        tr = np.load(os.path.join(args.data, "kh_train.npz"))
+        parts = sorted(glob.glob(os.path.join(args.data, "kh_ref_*.npz")))
+        refs = [np.load(p) for p in parts]
+        Yref = np.concatenate([r["ref"] for r in refs])[:args.n_ref]
+        seconds = float(tr["train_seconds"] + tr["holdout_seconds"] + sum(float(r["seconds"]) for r in refs))
+        count = tr["train"].shape[0] + tr["holdout"].shape[0] + sum(r["ref"].shape[0] for r in refs)
+        data = {"train": tr["train"], "holdout": tr["holdout"], "ref": Yref,
+                "solver_seconds_per_sample": seconds / count}
+        res, _ = uq_experiment(n_train=tr["train"].shape[0], n_holdout=tr["holdout"].shape[0], n_ref=Yref.shape[0],
+                               n_query=args.n_query, epochs=args.epochs, lr=args.lr, data=data, log=log, **kw)
+        print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()
