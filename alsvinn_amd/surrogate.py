@@ -265,7 +265,8 @@ def uq_experiment(n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), number_of_h=6
     def solver(X):
         start = time.perf_counter()
         # fallback samples are fresh query samples: their parameters are given
-        out = kh_observables(X, n=n, end_time=end_time, times=times, number_of_h=number_of_h, chunk=chunk)
+        out = kh_observables(X, n=n, end_time=end_time, times=times, number_of_h=number_of_h,
+                             chunk=min(chunk, 1024), chunk_callback=lambda k: log("fallback solver: %d samples" % k))
         log("fallback solver: %d samples in %.1f s" % (X.shape[0], time.perf_counter() - start))
         return out
 
