@@ -93,3 +93,23 @@ def test_monte_carlo_rate():
     rate_m = -np.polyfit(np.log(ms), np.log(errs_m), 1)[0]
     rate_v = -np.polyfit(np.log(ms), np.log(errs_v), 1)[0]
     assert rate_m >= 0.49 and rate_v >= 0.49
+
+
+def test_monte_carlo_rate_of_the_generator():
+    """MeanVarTest (test/library_tests/src/uq/generators/MeanVarTest.cpp:85-149):
+    40 uniform[0,1] components from a fresh generator for M = 2^4 .. 2^19
+    samples; the errors of the sample mean and the (N-1) sample variance
+    against 1/2 and 1/12 decay at rate >= 0.49 in M."""
+    from alsvinn_amd.uq import generate_parameters
+    logM, logEm, logEv = [], [], []
+    for k in range(4, 20):
+        M = 1 << k
+        x = generate_parameters([("a", 40, 0.0, 1.0)], 0, M)["a"]
+        em = np.sqrt(np.sum((x.mean(0) - 0.5) ** 2))
+        ev = np.sqrt(np.sum((x.var(0, ddof=1) - 1.0 / 12.0) ** 2))
+        logM.append(np.log(M))
+        logEm.append(np.log(em))
+        logEv.append(np.log(ev))
+    rate_mean = -np.polyfit(logM, logEm, 1)[0]
+    rate_var = -np.polyfit(logM, logEv, 1)[0]
+    assert rate_mean >= 0.49 and rate_var >= 0.49, (rate_mean, rate_var)
