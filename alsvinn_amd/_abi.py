@@ -46,6 +46,7 @@ SYMBOLS = [
     "avh_batch_set_kh2d",
     "avh_batch_advance",
     "avh_batch_get_progress",
+    "avh_batch_violations",
     "avh_batch_set_end_time",
     "avh_batch_kinetic_energy",
     "avh_batch_structure_cube",
@@ -162,6 +163,7 @@ def load():
         "avh_autotune": [vp, P(ctypes.c_int), P(d), P(ctypes.c_int)],
         "avh_batch_set_tile_shape": [vp, ctypes.c_int],
         "avh_batch_set_end_time": [vp, d],
+        "avh_batch_violations": [vp, ctypes.c_int, P(i64), P(i64)],
         "avh_batch_kinetic_energy": [vp, ctypes.c_int, ctypes.c_int, vp],
         "avh_batch_structure_cube": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, d, vp],
         "avh_batch_autotune": [vp, P(ctypes.c_int), P(d), P(ctypes.c_int)],

@@ -45,6 +45,7 @@ def run(path, device=0, out=sys.stdout):
     t0 = time.perf_counter()
     steps, left = br.run()
     t_run = time.perf_counter() - t0
+    br.check_constraints()
     files = []
     if st is not None:
         names = CONSERVED[cfg["equation"]]

@@ -774,6 +774,25 @@ avh_status avh_batch_set_graph_steps(avh_batch* b, int steps) {
     return AVH_OK;
 }
 
+avh_status avh_batch_violations(avh_batch* b, int group, int64_t* count, int64_t* first_sample) {
+    avh_status r = check_gs(b, group, 0);
+    if (r) return r;
+    if (!count || !first_sample) return fail(AVH_EINVAL, "null argument");
+    AVH_HIP(hipSetDevice(b->ctx->device));
+    AVH_HIP(hipStreamSynchronize(b->stream));
+    const GroupHost& G = b->g[group];
+    std::vector<Control> cs(G.nsamples);
+    AVH_HIP(hipMemcpy(cs.data(), G.ctrl, sizeof(Control) * G.nsamples, hipMemcpyDeviceToHost));
+    *count = 0;
+    *first_sample = -1;
+    for (int s = 0; s < G.nsamples; ++s)
+        if (cs[s].violated) {
+            if (*first_sample < 0) *first_sample = s;
+            *count += 1;
+        }
+    return AVH_OK;
+}
+
 avh_status avh_batch_set_end_time(avh_batch* b, double t_end) {
     if (!b) return fail(AVH_EINVAL, "null argument");
     AVH_HIP(hipSetDevice(b->ctx->device));

@@ -142,6 +142,7 @@ def kh_observables(params, n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), numb
         for k, t in enumerate(times):
             br.set_end_time(t)
             done, left = br.run()
+            br.check_constraints()
             if left:
                 raise RuntimeError("samples did not reach t = %g" % t)
             out[c0:c0 + p.shape[0], k] = br.kinetic_energy(0, 1)

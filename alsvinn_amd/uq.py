@@ -119,6 +119,24 @@ class BatchRunner:
                                                  ctypes.byref(left)))
         return done.value, left.value
 
+    def violations(self, group):
+        """(number of samples that failed the constraint check, first one or -1)."""
+        n = ctypes.c_int64()
+        first = ctypes.c_int64()
+        _abi.check(_abi.load().avh_batch_violations(self._h, int(group), ctypes.byref(n), ctypes.byref(first)))
+        return n.value, first.value
+
+    def check_constraints(self):
+        """Raise ConstraintError, as the reference's simulator does, if any
+        sample failed the constraint check."""
+        for g in range(len(self.groups)):
+            n, first = self.violations(g)
+            if n:
+                p = self.progress(g, first)
+                raise _abi.ConstraintError(
+                    "Simulation state does not obey constraints!  Group %d sample %d (and %d more) at time %.17g, "
+                    "number of timesteps performed %d" % (g, first, n - 1, p.time, p.steps))
+
     def set_end_time(self, t_end):
         """Every sample runs to t_end (again active if t < t_end)."""
         self.end_time = float(t_end)

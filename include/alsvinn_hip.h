@@ -227,6 +227,11 @@ avh_status avh_batch_set_kh2d(avh_batch* batch, int group, const double* host_in
 avh_status avh_batch_advance(avh_batch* batch, int64_t max_steps, int chunk, int64_t* steps_done,
                              int64_t* active_left);
 avh_status avh_batch_get_progress(avh_batch* batch, int group, int sample, avh_progress* progress);
+/* Samples of the group whose state failed the constraint check
+ * (Simulator::checkConstraints, alsfvm/src/simulator/Simulator.cpp:180-189,
+ * throws there; a batched sample stops instead): count and the first such
+ * sample (-1 if none). */
+avh_status avh_batch_violations(avh_batch* batch, int group, int64_t* count, int64_t* first_sample);
 /* Set every sample's end time (Simulator::setEndTime analogue): samples with
  * time < t_end and no constraint failure become active again.  With a
  * fixed-interval adjuster whose save times include t_end, every sample stops

@@ -135,3 +135,37 @@ def test_constant_fields_statistics_are_parameter_moments():
     assert cnt == M
     np.testing.assert_allclose(mean.ravel(), np.full(40, x.mean()), rtol=1e-13)
     np.testing.assert_allclose(var.ravel(), np.full(40, x.var()), rtol=1e-12)
+
+
+def test_violating_sample_stops_and_others_run_on():
+    """A sample whose pressure goes negative stops at the step where the
+    oracle raises; the batch reports it and check_constraints raises like
+    the reference; a healthy sample of the same batch runs on unaffected."""
+    from oracle import pyoracle
+    n = (32, 32)
+    bad = smooth_state("euler2", n, seed=5)
+    bad[3] = bad[3] - 0.999 * (bad[3] - 0.5 * (bad[1] ** 2 + bad[2] ** 2) / bad[0])
+    bad[1] += 3.0 * np.sign(np.sin(np.arange(n[0]) * 0.7))[None, :]
+    good = smooth_state("euler2", n, seed=6)
+    T = 0.05
+    br = BatchRunner("euler2", "weno2", "hll3", "rungekutta3", "periodic", [((32, 32, 1), 3)], T, cfl=0.9)
+    for s, q in enumerate([good, bad, good]):
+        br.set_state(0, s, q)
+    br.run()
+    assert br.violations(0) == (1, 1)
+    with pytest.raises(alsvinn_amd.ConstraintError):
+        br.check_constraints()
+    o = pyoracle.Simulator("euler2", "weno2", "hll3", "rungekutta3", "periodic", (32, 32, 1), cfl=0.9)
+    o.set_interior(bad)
+    steps = 0
+    with pytest.raises(pyoracle.OracleError):
+        for steps in range(1, 100):
+            o.step()
+    p = br.progress(0, 1)
+    assert p.violated and p.steps == steps
+    og = pyoracle.Simulator("euler2", "weno2", "hll3", "rungekutta3", "periodic", (32, 32, 1), cfl=0.9)
+    og.set_interior(good)
+    og.run(T)
+    for s in (0, 2):
+        np.testing.assert_array_equal(br.get_state(0, s), og.get_interior())
+        assert br.progress(0, s).time == og.time
