@@ -90,10 +90,12 @@ __device__ __forceinline__ void recon_face_reg(const double (&u)[WZ][NV], const 
     }
 }
 
-template <class EQ, int RECON, class FLUX, int TX, int TY, int LZ>
+template <class EQ, int RECON, class FLUX, int TX, int TY, int LZ, int NTH>
 struct MarchKernel3 {
     static constexpr int NV = EQ::NV;
-    static constexpr int NT = TX * TY;  // one column per thread
+    static constexpr int NC = TX * TY;  // columns: one per thread, threads past them work on the planes only
+    static constexpr int NT = NTH;
+    static_assert(NT >= NC && NT % 64 == 0, "one thread per column, whole wavefronts");
     static constexpr int G = ReconTraits<RECON>::GHOST;
     static constexpr int HW = G - 1 > 1 ? G - 1 : 1;  // z-window half width
     static constexpr int WZ = 2 * HW + 1;              // planes k+1-HW .. k+1+HW
@@ -104,6 +106,10 @@ struct MarchKernel3 {
     static constexpr int NIY = TX * (TY + 1);
     static constexpr int NIMAX = NIX > NIY ? NIX : NIY;
     static constexpr int NHALO = LP - TX * TY;
+
+    __host__ __device__ static unsigned int tiles(const int* n) {
+        return (unsigned int)(((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + LZ - 1) / LZ));
+    }
 
     __device__ __forceinline__ static long long gidx(int x, int y, int z, int n0, int n1) {
         return ((long long)z * n1 + y) * n0 + x;
@@ -171,51 +177,62 @@ struct MarchKernel3 {
         // this thread's column.  A column past the grid edge (partial tile)
         // sits where the tile's x/y halo lies, so it loads the boundary image
         // of that position, exactly as the halo ring does.
-        const int cx = threadIdx.x % TX, cy = threadIdx.x / TX;
-        const bool col_ok = (ox + cx < n0) && (oy + cy < n1);
+        const bool has_col = (int)threadIdx.x < NC;
+        const int cx = threadIdx.x % TX, cy = has_col ? (int)threadIdx.x / TX : 0;
+        const bool col_ok = has_col && (ox + cx < n0) && (oy + cy < n1);
         const int gxc = bc_map(ox + cx < n0 + G ? ox + cx : n0 + G - 1, n0, P.bc);
         const int gyc = bc_map(oy + cy < n1 + G ? oy + cy : n1 + G - 1, n1, P.bc);
 
         double ws[3] = {0.0, 0.0, 0.0};
         double w[WZ][NV];
         double rprev[NV], fprev[NV];
+#pragma unroll
+        for (int i = 0; i < WZ; ++i)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) w[i][v] = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) rprev[v] = fprev[v] = 0.0;
 
         // prologue: window centred at z0-1 -> R(z0-1); then centred at z0 ->
         // L(z0), R(z0), F_z(z0-1/2); then centred at z0+1 for the loop
+        if (has_col) {
 #pragma unroll
-        for (int i = 0; i < WZ; ++i) {
-            const int gz = bc_map(z0 - 1 - HW + i, n2, P.bc);
-            fetch(P, otf, s, gidx(gxc, gyc, gz, n0, n1), w[i]);
+            for (int i = 0; i < WZ; ++i) {
+                const int gz = bc_map(z0 - 1 - HW + i, n2, P.bc);
+                fetch(P, otf, s, gidx(gxc, gyc, gz, n0, n1), w[i]);
+            }
+            double rm1[NV];
+            recon_face_reg<RECON, NV, true, WZ>(w, HW, rm1);
+#pragma unroll
+            for (int i = 0; i < WZ - 1; ++i)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
+            fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
+            {
+                double l0[NV];
+                recon_face_reg<RECON, NV, false, WZ>(w, HW, l0);
+                recon_face_reg<RECON, NV, true, WZ>(w, HW, rprev);
+                const typename EQ::All aL = EQ::make_all(rm1, P.gamma);
+                const typename EQ::All aR = EQ::make_all(l0, P.gamma);
+                const double wz = FLUX::template flux<2, EQ>(aL, aR, P.gamma, fprev);
+                if (col_ok) ws[2] = ws_max(ws[2], wz);
+            }
+#pragma unroll
+            for (int i = 0; i < WZ - 1; ++i)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
+            fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + 1 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
         }
-        double rm1[NV];
-        recon_face_reg<RECON, NV, true, WZ>(w, HW, rm1);
-#pragma unroll
-        for (int i = 0; i < WZ - 1; ++i)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
-        fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
-        {
-            double l0[NV];
-            recon_face_reg<RECON, NV, false, WZ>(w, HW, l0);
-            recon_face_reg<RECON, NV, true, WZ>(w, HW, rprev);
-            const typename EQ::All aL = EQ::make_all(rm1, P.gamma);
-            const typename EQ::All aR = EQ::make_all(l0, P.gamma);
-            const double wz = FLUX::template flux<2, EQ>(aL, aR, P.gamma, fprev);
-            if (col_ok) ws[2] = ws_max(ws[2], wz);
-        }
-#pragma unroll
-        for (int i = 0; i < WZ - 1; ++i)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
-        fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + 1 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
 
         bool bad = false;
         const bool final_stage = (P.mode == MODE_RK2_2 || P.mode == MODE_RK3_3);
         for (int k = z0; k < z1; ++k) {
             // plane k into LDS: own column from the window (w[HW-1] is plane k),
             // the halo ring through the boundary map
+            if (has_col) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) sp[v * LP + (cy + G) * LXP + (cx + G)] = w[HW - 1][v];
+                for (int v = 0; v < NV; ++v) sp[v * LP + (cy + G) * LXP + (cx + G)] = w[HW - 1][v];
+            }
             for (int h = threadIdx.x; h < NHALO; h += NT) {
                 // enumerate the ring: rows above and below the tile, then the side columns
                 int lx, ly;
@@ -248,7 +265,7 @@ struct MarchKernel3 {
             for (int v = 0; v < NV; ++v) acc[v] = 0.0;
             plane_pass<0>(sp, sf, ox, oy, P.n, P.gamma, ws[0]);
             __syncthreads();
-            {
+            if (has_col) {
                 const int ip = cy * (TX + 1) + cx + 1;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) acc[v] = acc[v] + ((-sf[v * NIMAX + ip]) + sf[v * NIMAX + ip - 1]);
@@ -256,14 +273,16 @@ struct MarchKernel3 {
             __syncthreads();
             plane_pass<1>(sp, sf, ox, oy, P.n, P.gamma, ws[1]);
             __syncthreads();
-            {
+            if (has_col) {
                 const int ip = (cy + 1) * TX + cx;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) acc[v] = acc[v] + ((-sf[v * NIMAX + ip]) + sf[v * NIMAX + ip - TX]);
             }
             // z: F(k+1/2) from R(k) (kept) and L(k+1) (window centred at k+1)
             double fz[NV], rnext[NV];
-            {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) fz[v] = rnext[v] = 0.0;
+            if (has_col) {
                 double lz[NV];
                 recon_face_reg<RECON, NV, false, WZ>(w, HW, lz);
                 recon_face_reg<RECON, NV, true, WZ>(w, HW, rnext);
@@ -317,7 +336,7 @@ struct MarchKernel3 {
             for (int i = 0; i < WZ - 1; ++i)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
-            if (k + 1 < z1) fetch(P, otf, s, gidx(gxc, gyc, bc_map(k + 2 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
+            if (has_col && k + 1 < z1) fetch(P, otf, s, gidx(gxc, gyc, bc_map(k + 2 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
             __syncthreads();
         }
 

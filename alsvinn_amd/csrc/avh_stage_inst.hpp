@@ -19,12 +19,21 @@ typedef hipError_t (*BatchLauncher)(const BatchParams& B, unsigned int nblocks, 
 //   1D: (cells, threads) = (256, 256) or (64, 64): small grids get more
 //       workgroups from the smaller tile.
 //   2D: 32x16 (LDS 40.8 KB, 4 waves/SIMD for euler2), 32x8 (23 KB, 5
-//       waves/SIMD) or 64x4 cells, 256 threads.
-//   3D: 32x8 columns marching over 32 planes, 256 threads.
-constexpr int NSHAPES1 = 2, NSHAPES2 = 3, NSHAPES3 = 1;
+//       waves/SIMD), 64x4, 32x15 or 24x20 cells, 256 threads.  A flux pass
+//       hands interface items to threads in turns of 256, so a pass with a
+//       few items past a multiple of 256 gives wave 0 -- which sits on SIMD 0
+//       in every workgroup -- an extra turn: 32x16 has 528 x- and 544
+//       y-interfaces.  32x15 (495 / 512) and 24x20 (500 / 504) take exactly
+//       two turns on every wave.
+//   3D: columns marching over 32 planes, 256 threads: 32x8 (264 / 288
+//       plane interfaces), 16x15 or 32x7 (at most 256 each: one turn; the
+//       threads past the columns only work on the planes).
+constexpr int NSHAPES1 = 2, NSHAPES2 = 5, NSHAPES3 = 3;
 constexpr int SHAPE1_TX[NSHAPES1] = {256, 64};
-constexpr int SHAPE2_TX[NSHAPES2] = {32, 32, 64};
-constexpr int SHAPE2_TY[NSHAPES2] = {16, 8, 4};
+constexpr int SHAPE2_TX[NSHAPES2] = {32, 32, 64, 32, 24};
+constexpr int SHAPE2_TY[NSHAPES2] = {16, 8, 4, 15, 20};
+constexpr int SHAPE3_TX[NSHAPES3] = {32, 16, 32};
+constexpr int SHAPE3_TY[NSHAPES3] = {8, 15, 7};
 
 inline int num_shapes(int dim) { return dim == 1 ? NSHAPES1 : (dim == 2 ? NSHAPES2 : NSHAPES3); }
 
@@ -67,6 +76,8 @@ hipError_t launch_batch_stage(const BatchParams& B, unsigned int nblocks, int sh
         case 0: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 0>::type>(B, nblocks, stream);
         case 1: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 1>::type>(B, nblocks, stream);
         case 2: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 2>::type>(B, nblocks, stream);
+        case 3: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 3>::type>(B, nblocks, stream);
+        case 4: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 4>::type>(B, nblocks, stream);
         }
         return hipErrorInvalidValue;
     }
@@ -78,14 +89,21 @@ hipError_t launch_stage_k(const StageParams& P, hipStream_t stream) {
     return hipGetLastError();
 }
 
+template <class K>
+hipError_t launch_march_k(const StageParams& P, hipStream_t stream) {
+    hipLaunchKernelGGL(march_kernel3<K>, dim3(K::tiles(P.n)), dim3(K::NT), 0, stream, P);
+    return hipGetLastError();
+}
+
 template <class EQ, int RECON, class FLUX, int DIM>
 hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
     if constexpr (DIM == 3) {
-        if (shape != 0) return hipErrorInvalidValue;
-        typedef MarchKernel3<EQ, RECON, FLUX, 32, 8, 32> K;
-        const unsigned int nb = ((P.n[0] + 31) / 32) * ((P.n[1] + 7) / 8) * ((P.n[2] + 31) / 32);
-        hipLaunchKernelGGL(march_kernel3<K>, dim3(nb), dim3(K::NT), 0, stream, P);
-        return hipGetLastError();
+        switch (shape) {
+        case 0: return launch_march_k<MarchKernel3<EQ, RECON, FLUX, SHAPE3_TX[0], SHAPE3_TY[0], 32, 256>>(P, stream);
+        case 1: return launch_march_k<MarchKernel3<EQ, RECON, FLUX, SHAPE3_TX[1], SHAPE3_TY[1], 32, 256>>(P, stream);
+        case 2: return launch_march_k<MarchKernel3<EQ, RECON, FLUX, SHAPE3_TX[2], SHAPE3_TY[2], 32, 256>>(P, stream);
+        }
+        return hipErrorInvalidValue;
     } else if constexpr (DIM == 1) {
         switch (shape) {
         case 0: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 1, 0>::type>(P, stream);
@@ -97,6 +115,8 @@ hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
         case 0: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 0>::type>(P, stream);
         case 1: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 1>::type>(P, stream);
         case 2: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 2>::type>(P, stream);
+        case 3: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 3>::type>(P, stream);
+        case 4: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 4>::type>(P, stream);
         }
         return hipErrorInvalidValue;
     }

@@ -19,7 +19,8 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0
-SHAPE_NAMES = {1: ["256", "64"], 2: ["32x16", "32x8", "64x4"], 3: ["32x8 columns x 32 planes"]}
+SHAPE_NAMES = {1: ["256", "64"], 2: ["32x16", "32x8", "64x4", "32x15", "24x20"],
+               3: ["32x8 columns x 32 planes", "16x15 columns x 32 planes", "32x7 columns x 32 planes"]}
 
 WORKLOADS = {
     # the bench line
@@ -88,6 +89,44 @@ def cpu_baseline(wl, seconds_budget=15.0):
     }
 
 
+def run_uq_to_end(args, br, cells, total):
+    """The whole Monte Carlo run: every sample to T = 2, statistics on the
+    device; reports the wall time and the statistics' mean over the domain."""
+    import numpy as np
+    t0 = time.perf_counter()
+    done = 0
+    while True:
+        d, left = br.run(max_steps=500, chunk=100)
+        done += d
+        print("bench: %d batched steps, %d samples running, %.1f s" % (done, left, time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
+        if left == 0:
+            break
+    wall = time.perf_counter() - t0
+    br.check_constraints()
+    steps = [[br.progress(g, s).steps for s in range(m)] for g, (n, m) in enumerate(UQ_GROUPS)]
+    updates = sum(n[0] * n[1] * sum(st) for (n, m), st in zip(UQ_GROUPS, steps))
+    stats = {}
+    for g, (n, m) in enumerate(UQ_GROUPS):
+        mean, var, cnt = br.statistics(g, 0, primitive=True)
+        stats["%dx%d" % n[:2]] = {"samples": int(cnt), "steps_min_max": [min(steps[g]), max(steps[g])],
+                                  "domain_mean_of_mean_rho_ux_uy_p": [float(v) for v in mean.mean(axis=(1, 2, 3))],
+                                  "domain_mean_of_variance_rho_ux_uy_p": [float(v) for v in var.mean(axis=(1, 2, 3))]}
+    return {
+        "metric": "UQ wall time (all samples to T = 2, on-device mean/variance)",
+        "value": wall,
+        "unit": "s",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": UQ_DESC, "cells": cells, "samples": total, "parallelism": "single GPU"},
+        "cell_updates": updates,
+        "cell_updates_per_s": updates / wall,
+        "statistics": stats,
+    }
+
+
 def run_uq(args):
     import alsvinn_amd
     from alsvinn_amd import BatchRunner, generate_parameters
@@ -101,6 +140,8 @@ def run_uq(args):
         br.set_kelvin_helmholtz(g, params[k:k + m])
         k += m
     cells = sum(n[0] * n[1] * m for n, m in UQ_GROUPS)
+    if args.uq_to_end:
+        return run_uq_to_end(args, br, cells, total)
     best, tune_ms = br.autotune()
     tune = {"chosen": SHAPE_NAMES[2][best], "ms_per_shape": dict(zip(SHAPE_NAMES[2], tune_ms))}
     br.run(max_steps=args.warmup, chunk=max(1, args.warmup))
@@ -178,6 +219,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["uq"], default="kh2d")
+    ap.add_argument("--uq-to-end", action="store_true",
+                    help="with --workload uq: run every sample to T = 2 and report the wall time")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (torch copy kernel), to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters against a known byte count")

@@ -159,13 +159,14 @@ avh_status avh_set_graph_steps(avh_solver* solver, int steps);
 /* Tile shape of the fused stage kernel: -1 (default) lets the autotuner
  * pick on first use (cached per equation, reconstruction, flux and grid for
  * the process), 0..nshapes-1 fixes one (1D: 256 or 64 cells; 2D: 32x16,
- * 32x8 or 64x4; 3D: one shape).  Results are identical for every shape.
+ * 32x8, 64x4, 32x15 or 24x20; 3D: 32x8, 16x15 or 32x7 columns marching over
+ * 32 planes).  Results are identical for every shape.
  * No reference counterpart: the reference's CUDA kernels use fixed block
  * sizes (alsfvm/src/numflux/NumericalFluxCUDA.cu:330-414). */
 avh_status avh_set_tile_shape(avh_solver* solver, int shape);
 /* Time every tile shape on the solver's current state now (flux-only
  * launches; the solution and step state are unchanged) and select the
- * fastest.  ms_per_shape (may be NULL) receives *nshapes <= 4 timings. */
+ * fastest.  ms_per_shape (may be NULL) receives *nshapes <= 8 timings. */
 avh_status avh_autotune(avh_solver* solver, int* best_shape, double* ms_per_shape, int* nshapes);
 
 /* Run `steps` full time steps with a HIP event pair around every kernel

@@ -17,9 +17,11 @@ CASES = [
     # equation, recon, flux, integrator, bc, n, cfl, nshapes
     ("burgers", "weno2", "godunov", "rungekutta3", "periodic", (203, 1, 1), 0.45, 2),
     ("euler1", "eno3", "hll3", "rungekutta2", "neumann", (131, 1, 1), 0.4, 2),
-    ("euler2", "weno2", "hll3", "rungekutta3", "periodic", (70, 37, 1), 0.4, 3),
-    ("euler2", "eno2", "hll", "forwardeuler", "neumann", (45, 29, 1), 0.2, 3),
-    ("burgers", "none", "godunov", "rungekutta3", "periodic", (33, 66, 1), 0.4, 3),
+    ("euler2", "weno2", "hll3", "rungekutta3", "periodic", (70, 37, 1), 0.4, 5),
+    ("euler2", "eno2", "hll", "forwardeuler", "neumann", (45, 29, 1), 0.2, 5),
+    ("burgers", "none", "godunov", "rungekutta3", "periodic", (33, 66, 1), 0.4, 5),
+    ("euler3", "weno2", "hll3", "rungekutta3", "periodic", (35, 19, 40), 0.3, 3),
+    ("euler3", "eno3", "hll", "rungekutta2", "neumann", (17, 33, 36), 0.3, 3),
 ]
 
 
@@ -28,11 +30,11 @@ def _say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("case", CASES, ids=["-".join(map(str, c[:2])) + "-%dx%d" % c[5][:2] for c in CASES])
+@pytest.mark.parametrize("case", CASES, ids=["-".join(map(str, c[:2])) + "-%dx%dx%d" % c[5] for c in CASES])
 def test_every_tile_shape_matches_oracle(case):
     from oracle import pyoracle
     eq, recon, flux, integ, bc, n, cfl, ns = case
-    dim = 2 if n[1] > 1 else 1
+    dim = 3 if n[2] > 1 else (2 if n[1] > 1 else 1)
     q0 = smooth_state(eq, n[:dim], seed=11, jump=True)
     o = pyoracle.Simulator(eq, recon, flux, integ, bc, n, cfl=cfl)
     o.set_interior(q0)
@@ -62,7 +64,7 @@ def test_autotune_keeps_state_and_picks_a_shape():
     p0 = h.progress()
     best, ms = h.autotune()
     _say("autotune", best, ms)
-    assert 0 <= best < len(ms) == 3 and all(t > 0 for t in ms)
+    assert 0 <= best < len(ms) == 5 and all(t > 0 for t in ms)
     np.testing.assert_array_equal(h.get_interior(), before)
     assert h.progress().time == p0.time and h.progress().steps == p0.steps
     for _ in range(5):
@@ -77,7 +79,7 @@ def test_autotune_keeps_state_and_picks_a_shape():
 def test_batch_tile_shapes_agree():
     groups = [((40, 24, 1), 3), ((20, 20, 1), 2)]
     res = []
-    for shape in (0, 1, 2, -1):
+    for shape in (0, 1, 2, 3, 4, -1):
         br = BatchRunner("euler2", "weno2", "hll3", "rungekutta3", "periodic", groups, 0.02, cfl=0.4, stats_saves=1)
         br.set_tile_shape(shape)
         for g, (nn, m) in enumerate(groups):

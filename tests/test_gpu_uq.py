@@ -165,6 +165,7 @@ def test_violating_sample_stops_and_others_run_on():
     assert p.violated and p.steps == steps
     og = pyoracle.Simulator("euler2", "weno2", "hll3", "rungekutta3", "periodic", (32, 32, 1), cfl=0.9)
     og.set_interior(good)
+    og.add_fixed_interval(T)  # the batch's statistics save at T
     og.run(T)
     for s in (0, 2):
         np.testing.assert_array_equal(br.get_state(0, s), og.get_interior())
