@@ -292,18 +292,21 @@ def uq_experiment(n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), number_of_h=6
     budget = n_train + n_holdout
     res["plain_mean_err_at_budget"], res["plain_var_err_at_budget"] = plain_mc_error(Yref, ref_mean, ref_var,
                                                                                       min(budget, n_ref // 2))
-    # plain Monte Carlo samples for the surrogate's error (1/sqrt(N) scaling
-    # from the measured error at the budget), and its wall time
+    # fallback: the solver samples it used in total, and plain Monte Carlo's
+    # error with that many samples (subsets of the reference pool)
+    n_fb = budget + int(round(st["fallback_fraction"] * n_query))
+    res["fallback_solver_samples"] = n_fb
+    res["plain_mean_err_at_fallback_samples"], res["plain_var_err_at_fallback_samples"] = plain_mc_error(
+        Yref, ref_mean, ref_var, min(n_fb, n_ref // 2))
     wall_sur = budget * t_sample + t_train + st["predict_seconds"] + st["solver_seconds"]
     res["surrogate_wall_seconds"] = wall_sur
-    for name in ("fallback", "cv"):
-        e_m = max(res[name + "_mean_err"], 1e-300)
-        e_v = max(res[name + "_var_err"], 1e-300)
-        n_m = min(budget, n_ref // 2) * (res["plain_mean_err_at_budget"] / e_m) ** 2
-        n_v = min(budget, n_ref // 2) * (res["plain_var_err_at_budget"] / e_v) ** 2
-        n_eq = max(n_m, n_v)
-        res[name + "_plain_equivalent_samples"] = n_eq
-        res[name + "_speedup"] = n_eq * t_sample / wall_sur
+    # control variate: its variance is var(f - g) / n_holdout (+ var(g) /
+    # n_query), so it stands for n_holdout * var(f) / var(f - g) plain solver
+    # samples per observable; the speedup is that plain run's solver time over
+    # the whole surrogate pipeline's wall time (training samples included)
+    n_eq = n_holdout * float(np.median(eff))
+    res["cv_plain_equivalent_samples_median"] = n_eq
+    res["cv_speedup_median"] = n_eq * t_sample / (budget * t_sample + t_train + st["predict_seconds"])
     return res, data
 
 
