@@ -142,8 +142,12 @@ def run_uq(args):
     cells = sum(n[0] * n[1] * m for n, m in UQ_GROUPS)
     if args.uq_to_end:
         return run_uq_to_end(args, br, cells, total)
-    best, tune_ms = br.autotune()
-    tune = {"chosen": SHAPE_NAMES[2][best], "ms_per_shape": dict(zip(SHAPE_NAMES[2], tune_ms))}
+    if args.tile_shape >= 0:
+        br.set_tile_shape(args.tile_shape)
+        tune = {"chosen": SHAPE_NAMES[2][args.tile_shape], "fixed": True}
+    else:
+        best, tune_ms = br.autotune()
+        tune = {"chosen": SHAPE_NAMES[2][best], "ms_per_shape": dict(zip(SHAPE_NAMES[2], tune_ms))}
     br.run(max_steps=args.warmup, chunk=max(1, args.warmup))
     print("bench: warmup done", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -219,6 +223,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["uq"], default="kh2d")
+    ap.add_argument("--tile-shape", type=int, default=-1,
+                    help="fix the stage kernel's tile shape (index into the autotuner's list) instead of "
+                         "autotuning, e.g. to profile one kernel instance")
     ap.add_argument("--uq-to-end", action="store_true",
                     help="with --workload uq: run every sample to T = 2 and report the wall time")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -258,9 +265,13 @@ def main():
     print("bench: building %s state" % args.workload, file=sys.stderr, flush=True)
     sim = alsvinn_amd.Simulator(wl["equation"], "weno2", "hll3", "rungekutta3", "periodic", wl["n"], cfl=wl["cfl"])
     sim.set_interior(initial_state(wl))
-    best, tune_ms = sim.autotune()
     dim = 3 if wl["n"][2] > 1 else (2 if wl["n"][1] > 1 else 1)
-    tune = {"chosen": SHAPE_NAMES[dim][best], "ms_per_shape": dict(zip(SHAPE_NAMES[dim], tune_ms))}
+    if args.tile_shape >= 0:
+        sim.set_tile_shape(args.tile_shape)
+        tune = {"chosen": SHAPE_NAMES[dim][args.tile_shape], "fixed": True}
+    else:
+        best, tune_ms = sim.autotune()
+        tune = {"chosen": SHAPE_NAMES[dim][best], "ms_per_shape": dict(zip(SHAPE_NAMES[dim], tune_ms))}
 
     sim.enqueue_steps(args.warmup)
     sim.synchronize()
