@@ -303,13 +303,18 @@ def main():
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
 
     traffic = None
+    valu = None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                          "stage_kernel_%s_traffic.json" % WORKLOADS[args.workload]["profile"])
     if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+        prof = json.load(open(tfile))
+        traffic = prof.get("bytes_per_launch")
+        insts = prof.get("valu_insts_per_launch")
+        if insts:
+            # second bound: FP64 VALU issue, 1024 SIMDs x one wave64 instruction per 4 cycles at 2.4 GHz
+            peak = 1024 * 2.4e9 / 4
+            valu = {"insts_per_launch": insts, "achieved_per_s": insts / (avg_ms * 1e-3), "peak_per_s": peak,
+                    "frac": insts / (avg_ms * 1e-3) / peak, "source": os.path.basename(tfile)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -347,6 +352,7 @@ def main():
             "stage_ms": stage_ms,
             "end_of_step_kernel_ms": finish_ms,
             "tile_autotune": tune,
+            "valu": valu,
         },
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
