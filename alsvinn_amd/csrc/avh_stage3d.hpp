@@ -226,15 +226,21 @@ struct MarchKernel3 {
 
         bool bad = false;
         const bool final_stage = (P.mode == MODE_RK2_2 || P.mode == MODE_RK3_3);
-        for (int k = z0; k < z1; ++k) {
-            // plane k into LDS: own column from the window (w[HW-1] is plane k),
-            // the halo ring through the boundary map
-            if (has_col) {
+        // This thread's cells of the x/y halo ring (HPT per thread at most):
+        // their LDS slots and boundary-mapped positions in a plane do not
+        // change with k.  The ring rows above and below the tile come first,
+        // then the side columns.
+        constexpr int HPT = (NHALO + NT - 1) / NT;
+        int halo_lds[HPT];
+        long long halo_xy[HPT];
+        bool has_halo[HPT];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) sp[v * LP + (cy + G) * LXP + (cx + G)] = w[HW - 1][v];
-            }
-            for (int h = threadIdx.x; h < NHALO; h += NT) {
-                // enumerate the ring: rows above and below the tile, then the side columns
+        for (int j = 0; j < HPT; ++j) {
+            const int h = threadIdx.x + j * NT;
+            has_halo[j] = h < NHALO;
+            halo_lds[j] = 0;
+            halo_xy[j] = 0;
+            if (has_halo[j]) {
                 int lx, ly;
                 if (h < G * LXP) {
                     ly = h / LXP;
@@ -253,12 +259,52 @@ struct MarchKernel3 {
                 gy = gy < n1 + G ? gy : n1 + G - 1;
                 gx = bc_map(gx, n0, P.bc);
                 gy = bc_map(gy, n1, P.bc);
-                double q[NV];
-                fetch(P, otf, s, gidx(gx, gy, k, n0, n1), q);
+                halo_lds[j] = ly * LXP + lx;
+                halo_xy[j] = (long long)gy * n0 + gx;
+            }
+        }
+        const long long plane = (long long)n0 * n1;
+        // the ring of the next plane is loaded while the current plane is
+        // computed: raw input (and U0 for the on-the-fly stage) in registers
+        double pin[HPT][NV], pu0[HPT][NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) sp[v * LP + ly * LXP + lx] = q[v];
+        for (int j = 0; j < HPT; ++j)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) pin[j][v] = pu0[j][v] = 0.0;
+#define AVH_PREFETCH_RING(KP)                                                   \
+    _Pragma("unroll") for (int j = 0; j < HPT; ++j) {                           \
+        if (has_halo[j]) {                                                      \
+            const long long gi = (long long)(KP) * plane + halo_xy[j];          \
+            _Pragma("unroll") for (int v = 0; v < NV; ++v) {                    \
+                pin[j][v] = P.in[v * P.ncells + gi];                            \
+                if (otf) pu0[j][v] = P.u0[v * P.ncells + gi];                   \
+            }                                                                   \
+        }                                                                       \
+    }
+        AVH_PREFETCH_RING(z0)
+
+        for (int k = z0; k < z1; ++k) {
+            // plane k into LDS: own column from the window (w[HW-1] is plane k),
+            // the halo ring from the prefetched registers
+            if (has_col) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sp[v * LP + (cy + G) * LXP + (cx + G)] = w[HW - 1][v];
+            }
+#pragma unroll
+            for (int j = 0; j < HPT; ++j) {
+                if (has_halo[j]) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        double val = pin[j][v];
+                        if (otf) val = val * s + pu0[j][v];
+                        sp[v * LP + halo_lds[j]] = val;
+                    }
+                }
             }
             __syncthreads();
+            if (k + 1 < z1) {
+                AVH_PREFETCH_RING(k + 1)
+            }
 
             double acc[NV];
 #pragma unroll
@@ -340,6 +386,7 @@ struct MarchKernel3 {
             __syncthreads();
         }
 
+#undef AVH_PREFETCH_RING
         if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
             const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
