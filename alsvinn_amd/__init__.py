@@ -20,4 +20,4 @@ from .simulator import (  # noqa: F401
     resolve_integrator,
 )
 from . import initial_data  # noqa: F401
-from .uq import BatchRunner, generate_parameters  # noqa: F401
+from .uq import BatchRunner, MeanVariance, generate_parameters  # noqa: F401

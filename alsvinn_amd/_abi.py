@@ -56,6 +56,11 @@ SYMBOLS = [
     "avh_batch_time_step_kernels",
     "avh_batch_get_statistics",
     "avh_uq_generate",
+    "avh_stats_create",
+    "avh_stats_destroy",
+    "avh_stats_reset",
+    "avh_stats_accumulate",
+    "avh_stats_read",
     "avh_mlp_create",
     "avh_mlp_destroy",
     "avh_mlp_num_params",
@@ -182,6 +187,11 @@ def load():
         "avh_batch_get_progress": [vp, ctypes.c_int, ctypes.c_int, P(AvhProgress)],
         "avh_batch_get_statistics": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, P(i64)],
         "avh_uq_generate": [ctypes.c_int, P(ctypes.c_int), P(d), P(d), i64, i64, vp],
+        "avh_stats_create": [vp, ctypes.c_int, P(ctypes.c_int), d, ctypes.c_int, P(vp)],
+        "avh_stats_destroy": [vp],
+        "avh_stats_reset": [vp],
+        "avh_stats_accumulate": [vp, vp, ctypes.c_int],
+        "avh_stats_read": [vp, ctypes.c_int, ctypes.c_int, vp, vp, P(i64)],
         "avh_mlp_create": [vp, P(AvhMlpConfig), P(vp)],
         "avh_mlp_destroy": [vp],
         "avh_mlp_num_params": [vp, P(i64)],

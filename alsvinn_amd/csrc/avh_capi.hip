@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <limits>
 #include <new>
@@ -122,6 +123,61 @@ void launch_ew(const double* L, const double* u0, double* out, Control* c, long 
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(ew_kernel<EQ>, dim3((unsigned)nb), dim3(256), 0, st, L, u0, out, c, nc, mode,
                        gamma);
+}
+
+// One sample folded into running mean / M2 arrays (Welford, n = samples
+// including this one), for the conserved variables and for rho, u, p
+// (MeanVariance, alsuq/src/stats/MeanVariance.cpp:30-67).
+template <class EQ>
+__global__ __launch_bounds__(256) void welford_kernel(const double* q, long long nc, double n, double gamma,
+                                                      double* mean_c, double* m2_c, double* mean_p, double* m2_p) {
+    constexpr int NV = EQ::NV;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nc) return;
+    double u[NV], p[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) u[v] = q[v * nc + i];
+    if constexpr (EQ::NV > 1) {
+        const typename EQ::All a = EQ::make_all(u, gamma);
+        p[0] = u[0];
+#pragma unroll
+        for (int v = 1; v < NV - 1; ++v) p[v] = a.u[v - 1];
+        p[NV - 1] = a.p;
+    } else {
+        p[0] = u[0];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const long long o = v * nc + i;
+        double m = mean_c[o];
+        const double d = u[v] - m;
+        m = m + d / n;
+        mean_c[o] = m;
+        m2_c[o] = m2_c[o] + d * (u[v] - m);
+        double mp = mean_p[o];
+        const double dp = p[v] - mp;
+        mp = mp + dp / n;
+        mean_p[o] = mp;
+        m2_p[o] = m2_p[o] + dp * (p[v] - mp);
+    }
+}
+
+typedef void (*WelfordLauncher)(const double*, long long, double, double, double*, double*, double*, double*,
+                                hipStream_t);
+template <class EQ>
+void launch_welford(const double* q, long long nc, double n, double gamma, double* a, double* b, double* c, double* d,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(welford_kernel<EQ>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, q, nc, n, gamma, a, b,
+                       c, d);
+}
+WelfordLauncher welford_for(int eq) {
+    switch (eq) {
+    case AVH_EQ_BURGERS: return &launch_welford<BurgersEq>;
+    case AVH_EQ_EULER1: return &launch_welford<EulerEq<1>>;
+    case AVH_EQ_EULER2: return &launch_welford<EulerEq<2>>;
+    case AVH_EQ_EULER3: return &launch_welford<EulerEq<3>>;
+    }
+    return nullptr;
 }
 
 EwLauncher ew_for(int eq) {
@@ -661,6 +717,108 @@ avh_status avh_advance(avh_solver* s, double t_end, int64_t max_steps, int chunk
     }
     fill_progress(s, progress);
     if (s->h.violated) return constraint_error(s);
+    return AVH_OK;
+}
+
+}  // extern "C"
+
+// Device-side meanvar statistics for callers that run one solver per sample
+// (the reference's Runner, alsuq/src/run/Runner.cpp:39-90): every save keeps
+// a sample count and mean / M2 arrays of the conserved and the primitive
+// variables.
+struct avh_stats {
+    avh_ctx* ctx = nullptr;
+    int equation = 0, nv = 0, nsaves = 0;
+    long long ncells = 0;
+    double gamma = 1.4;
+    std::vector<long long> count;
+    double* mem = nullptr;  // [4][nsaves][nv][ncells]: mean_c, m2_c, mean_p, m2_p
+    hipStream_t stream = nullptr;
+};
+
+extern "C" {
+
+avh_status avh_stats_create(avh_ctx* ctx, int equation, const int n[3], double gamma, int nsaves, avh_stats** out) {
+    if (!ctx || !n || !out) return fail(AVH_EINVAL, "null argument");
+    *out = nullptr;
+    const int nv = nvars_of(equation);
+    if (!nv) return fail(AVH_EINVAL, "unknown equation");
+    if (nsaves < 1 || n[0] < 1 || n[1] < 1 || n[2] < 1) return fail(AVH_EINVAL, "bad sizes");
+    AVH_HIP(hipSetDevice(ctx->device));
+    auto t = new (std::nothrow) avh_stats;
+    if (!t) return fail(AVH_ENOMEM, "host allocation failed");
+    t->ctx = ctx;
+    t->equation = equation;
+    t->nv = nv;
+    t->nsaves = nsaves;
+    t->ncells = (long long)n[0] * n[1] * n[2];
+    t->gamma = gamma;
+    t->count.assign(nsaves, 0);
+    const size_t bytes = sizeof(double) * 4 * (size_t)nsaves * nv * t->ncells;
+    if (hipMalloc(&t->mem, bytes) != hipSuccess) {
+        delete t;
+        return fail(AVH_ENOMEM, "device allocation failed");
+    }
+    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMemset(t->mem, 0, bytes) != hipSuccess) {
+        (void)hipFree(t->mem);
+        if (t->stream) (void)hipStreamDestroy(t->stream);
+        delete t;
+        return fail(AVH_EHIP, "device initialisation failed");
+    }
+    *out = t;
+    return AVH_OK;
+}
+
+avh_status avh_stats_destroy(avh_stats* t) {
+    if (!t) return AVH_OK;
+    (void)hipSetDevice(t->ctx->device);
+    (void)hipStreamSynchronize(t->stream);
+    (void)hipFree(t->mem);
+    (void)hipStreamDestroy(t->stream);
+    delete t;
+    return AVH_OK;
+}
+
+avh_status avh_stats_reset(avh_stats* t) {
+    if (!t) return fail(AVH_EINVAL, "null argument");
+    AVH_HIP(hipSetDevice(t->ctx->device));
+    AVH_HIP(hipStreamSynchronize(t->stream));
+    AVH_HIP(hipMemset(t->mem, 0, sizeof(double) * 4 * (size_t)t->nsaves * t->nv * t->ncells));
+    std::fill(t->count.begin(), t->count.end(), 0);
+    return AVH_OK;
+}
+
+avh_status avh_stats_accumulate(avh_stats* t, avh_solver* s, int save) {
+    if (!t || !s) return fail(AVH_EINVAL, "null argument");
+    if (save < 0 || save >= t->nsaves) return fail(AVH_EINVAL, "no such statistics save");
+    if (s->cfg.equation != t->equation || s->ncells != t->ncells) return fail(AVH_EINVAL, "solver does not match");
+    AVH_HIP(hipSetDevice(t->ctx->device));
+    // the solver's state is final once its stream is idle
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    const long long n = ++t->count[save];
+    const size_t blk = (size_t)t->nsaves * t->nv * t->ncells;
+    const size_t off = (size_t)save * t->nv * t->ncells;
+    welford_for(t->equation)(s->A, t->ncells, (double)n, t->gamma, t->mem + off, t->mem + blk + off,
+                             t->mem + 2 * blk + off, t->mem + 3 * blk + off, t->stream);
+    AVH_HIP(hipGetLastError());
+    AVH_HIP(hipStreamSynchronize(t->stream));
+    return AVH_OK;
+}
+
+avh_status avh_stats_read(avh_stats* t, int save, int primitive, double* mean, double* variance, int64_t* count) {
+    if (!t || !mean || !variance || !count) return fail(AVH_EINVAL, "null argument");
+    if (save < 0 || save >= t->nsaves) return fail(AVH_EINVAL, "no such statistics save");
+    AVH_HIP(hipSetDevice(t->ctx->device));
+    AVH_HIP(hipStreamSynchronize(t->stream));
+    const size_t blk = (size_t)t->nsaves * t->nv * t->ncells;
+    const size_t n = (size_t)t->nv * t->ncells;
+    const size_t off = (size_t)save * n + (primitive ? 2 * blk : 0);
+    AVH_HIP(hipMemcpy(mean, t->mem + off, n * sizeof(double), hipMemcpyDeviceToHost));
+    AVH_HIP(hipMemcpy(variance, t->mem + blk + off, n * sizeof(double), hipMemcpyDeviceToHost));
+    const long long c = t->count[save];
+    for (size_t i = 0; i < n; ++i) variance[i] = c > 0 ? variance[i] / (double)c : 0.0;
+    *count = c;
     return AVH_OK;
 }
 

@@ -3,6 +3,8 @@
   generate_parameters(...)   SampleGenerator + STLMersenne + Uniform
                              (alsuq/src/samples/SampleGenerator.cpp:29-56,
                               alsuq/src/generator/STLMersenne.cpp:25-85)
+  MeanVariance               device meanvar statistics fed one solver state at a
+                             time (alsuq/src/stats/MeanVariance.cpp:30-67)
   BatchRunner                Runner (alsuq/src/run/Runner.cpp:39-90) with all
                              samples of up to 4 resolution groups advanced in
                              the same launches, meanvar statistics on device
@@ -38,6 +40,43 @@ def generate_parameters(parameters, sample_start, nsamples):
         res[p[0]] = out[:, o:o + int(p[1])].copy()
         o += int(p[1])
     return res
+
+
+class MeanVariance:
+    """Welford mean / variance of solver states on the device, one save
+    index per statistics time (conserved variables and rho, u, p)."""
+
+    def __init__(self, equation, n, gamma=1.4, nsaves=1, device=0):
+        self.equation = equation
+        self.nv = NVARS[equation]
+        self.n = tuple(int(v) for v in n) + (1,) * (3 - len(n))
+        self.ctx = default_context(device)
+        dims = (ctypes.c_int * 3)(*self.n)
+        h = ctypes.c_void_p()
+        _abi.check(_abi.load().avh_stats_create(self.ctx._h, _lookup(EQUATIONS, equation, "equation"), dims,
+                                                float(gamma), int(nsaves), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _abi._lib is not None:
+            _abi._lib.avh_stats_destroy(h)
+            self._h = None
+
+    def reset(self):
+        _abi.check(_abi.load().avh_stats_reset(self._h))
+
+    def accumulate(self, simulator, save=0):
+        _abi.check(_abi.load().avh_stats_accumulate(self._h, simulator._h, int(save)))
+
+    def read(self, save=0, primitive=False):
+        shape = (self.nv, self.n[2], self.n[1], self.n[0])
+        mean = np.empty(shape)
+        var = np.empty(shape)
+        cnt = ctypes.c_int64()
+        _abi.check(_abi.load().avh_stats_read(self._h, int(save), int(bool(primitive)), mean.ctypes.data,
+                                              var.ctypes.data, ctypes.byref(cnt)))
+        return mean, var, cnt.value
 
 
 class BatchRunner:

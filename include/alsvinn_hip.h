@@ -278,6 +278,26 @@ avh_status avh_uq_generate(int nparams, const int* dims, const double* lower, co
                            int64_t sample_start, int64_t nsamples, double* out);
 
 /* ---------------------------------------------------------------------------
+ * Device-side meanvar statistics for one-solver-per-sample runs: the hip
+ * MeanVariance statistics (alsuq/src/stats/MeanVariance.cpp:30-67) a
+ * REGISTER_STATISTICS(hip, meanvar, ...) class would wrap.  Each save keeps
+ * a sample count and Welford mean / M2 arrays of the conserved variables and
+ * of rho, u, p, all interior-only SoA.
+ * ------------------------------------------------------------------------- */
+typedef struct avh_stats avh_stats;
+avh_status avh_stats_create(avh_ctx* ctx, int equation, const int n[3], double gamma, int nsaves,
+                            avh_stats** stats);
+avh_status avh_stats_destroy(avh_stats* stats);
+/* zero every save (count, mean, M2) */
+avh_status avh_stats_reset(avh_stats* stats);
+/* fold the solver's current state into save `save` */
+avh_status avh_stats_accumulate(avh_stats* stats, avh_solver* solver, int save);
+/* mean and population variance (M2 / count) of save `save`, conserved
+ * (primitive = 0) or rho, u, p (primitive = 1) */
+avh_status avh_stats_read(avh_stats* stats, int save, int primitive, double* mean, double* variance,
+                          int64_t* count);
+
+/* ---------------------------------------------------------------------------
  * Surrogate network: an ensemble of multilayer perceptrons n_in -> width
  * (GELU) x hidden -> n_out, bf16 weights and activations on the matrix
  * cores with fp32 accumulation, fp32 master weights trained by Adam on the

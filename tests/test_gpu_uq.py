@@ -170,3 +170,48 @@ def test_violating_sample_stops_and_others_run_on():
     for s in (0, 2):
         np.testing.assert_array_equal(br.get_state(0, s), og.get_interior())
         assert br.progress(0, s).time == og.time
+
+
+def test_device_meanvar_of_single_solvers_matches_batch():
+    """Per-sample solvers folded into the device statistics one at a time
+    (the Runner's one-simulator-per-sample path) give the naive statistics of
+    their states and the batch engine's statistics of the same samples."""
+    n, m, T = 24, 5, 0.03
+    params = generate_parameters([("a", 40, 0.0, 1.0)], 50, m)["a"]
+    st = alsvinn_amd.MeanVariance("euler2", (n, n), nsaves=2)
+    br = BatchRunner("euler2", "weno2", "hll3", "rungekutta3", "periodic", [((n, n, 1), m)], T, cfl=0.4,
+                     stats_saves=1, stats_write_initial=True)
+    br.set_kelvin_helmholtz(0, params)
+    br.run()
+    finals = []
+    for s in range(m):
+        sim = alsvinn_amd.Simulator("euler2", "weno2", "hll3", "rungekutta3", "periodic", (n, n, 1), cfl=0.4)
+        sim.set_interior(kelvin_helmholtz_2d(n, params[s]))
+        sim.add_fixed_interval_writer(T, 0.0, True)  # the statistics' save times
+        sim.ack_saves()
+        st.accumulate(sim, 0)
+        sim.run(T)
+        st.accumulate(sim, 1)
+        finals.append(sim.get_interior())
+        np.testing.assert_array_equal(finals[-1], br.get_state(0, s))
+    for k in (0, 1):
+        mean, var, cnt = st.read(k)
+        bm, bv, bc = br.statistics(0, k)
+        assert cnt == bc == m
+        pm, pv, _ = st.read(k, primitive=True)
+        bpm, bpv, _ = br.statistics(0, k, primitive=True)
+        if k == 0:
+            # every sample arrives at t = 0 together: the batch folds them in
+            # sample order, as the sequential accumulation does
+            for a, b in ((mean, bm), (var, bv), (pm, bpm), (pv, bpv)):
+                np.testing.assert_array_equal(a, b)
+        else:
+            # samples reach T after different step counts, so the batch folds
+            # them in arrival order
+            for a, b in ((mean, bm), (pm, bpm)):
+                np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-15)
+            for a, b, mm in ((var, bv, bm), (pv, bpv, bpm)):
+                assert np.all(np.abs(a - b) <= 64 * m * EPS * (mm * mm + np.abs(b)) + 1e-300)
+    check_stats(st.read(1)[0], st.read(1)[1], finals)
+    st.reset()
+    assert st.read(0)[2] == 0
