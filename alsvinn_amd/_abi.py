@@ -38,6 +38,7 @@ SYMBOLS = [
     "avh_time_step_kernels",
     "avh_stream",
     "avh_compute_flux",
+    "avh_reconstruct",
     "avh_batch_create",
     "avh_batch_destroy",
     "avh_batch_add_fixed_interval_adjuster",
@@ -176,6 +177,7 @@ def load():
         "avh_batch_time_step_kernels": [vp, i64, P(d), P(ctypes.c_int)],
         "avh_synchronize": [vp],
         "avh_compute_flux": [vp, vp, P(d)],
+        "avh_reconstruct": [vp, ctypes.c_int, vp, vp],
         "avh_time_step_kernels": [vp, i64, P(d), P(ctypes.c_int)],
         "avh_batch_create": [vp, P(AvhConfig), ctypes.c_int, P(AvhGroup), d, ctypes.c_int, ctypes.c_int, P(vp)],
         "avh_batch_destroy": [vp],

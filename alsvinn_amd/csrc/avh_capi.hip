@@ -162,6 +162,60 @@ __global__ __launch_bounds__(256) void welford_kernel(const double* q, long long
     }
 }
 
+// Unfused reconstruction along one direction: the left- and right-face
+// values of every interior cell (Reconstruction::performReconstruction,
+// alsfvm/src/reconstruction/ReconstructionCPU.cpp:37-97, with the ghost
+// cells taken through the boundary map).  One thread per cell; the stencil
+// along `dir` is gathered into a register window centred on the cell.
+template <int RECON, int NV>
+__global__ __launch_bounds__(256) void reconstruct_kernel(const double* in, double* L, double* R, long long nc, int n0,
+                                                          int n1, int n2, int dir, int bc) {
+    constexpr int G = ReconTraits<RECON>::GHOST;
+    constexpr int HW = G - 1 > 1 ? G - 1 : 1;
+    constexpr int WZ = 2 * HW + 1;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nc) return;
+    const int x = (int)(i % n0), y = (int)((i / n0) % n1), z = (int)(i / ((long long)n0 * n1));
+    double u[WZ][NV];
+#pragma unroll
+    for (int k = 0; k < WZ; ++k) {
+        int xx = x, yy = y, zz = z;
+        if (dir == 0) xx = bc_map(x + k - HW, n0, bc);
+        else if (dir == 1) yy = bc_map(y + k - HW, n1, bc);
+        else zz = bc_map(z + k - HW, n2, bc);
+        const long long gi = ((long long)zz * n1 + yy) * n0 + xx;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) u[k][v] = in[v * nc + gi];
+    }
+    double l[NV], r[NV];
+    recon_face_reg<RECON, NV, false, WZ>(u, HW, l);
+    recon_face_reg<RECON, NV, true, WZ>(u, HW, r);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        L[v * nc + i] = l[v];
+        R[v * nc + i] = r[v];
+    }
+}
+
+template <int RECON, int NV>
+void launch_reconstruct(const double* in, double* L, double* R, long long nc, const int* n, int dir, int bc,
+                        hipStream_t st) {
+    hipLaunchKernelGGL((reconstruct_kernel<RECON, NV>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, in, L,
+                       R, nc, n[0], n[1], n[2], dir, bc);
+}
+
+template <int NV>
+bool launch_reconstruct_nv(int recon, const double* in, double* L, double* R, long long nc, const int* n, int dir,
+                           int bc, hipStream_t st) {
+    switch (recon) {
+    case RECON_NONE: launch_reconstruct<RECON_NONE, NV>(in, L, R, nc, n, dir, bc, st); return true;
+    case RECON_WENO2: launch_reconstruct<RECON_WENO2, NV>(in, L, R, nc, n, dir, bc, st); return true;
+    case RECON_ENO2: launch_reconstruct<RECON_ENO2, NV>(in, L, R, nc, n, dir, bc, st); return true;
+    case RECON_ENO3: launch_reconstruct<RECON_ENO3, NV>(in, L, R, nc, n, dir, bc, st); return true;
+    }
+    return false;
+}
+
 typedef void (*WelfordLauncher)(const double*, long long, double, double, double*, double*, double*, double*,
                                 hipStream_t);
 template <class EQ>
@@ -819,6 +873,29 @@ avh_status avh_stats_read(avh_stats* t, int save, int primitive, double* mean, d
     const long long c = t->count[save];
     for (size_t i = 0; i < n; ++i) variance[i] = c > 0 ? variance[i] / (double)c : 0.0;
     *count = c;
+    return AVH_OK;
+}
+
+avh_status avh_reconstruct(avh_solver* s, int direction, double* host_left, double* host_right) {
+    if (!s || !host_left || !host_right) return fail(AVH_EINVAL, "null argument");
+    if (direction < 0 || direction >= s->dim) return fail(AVH_EINVAL, "direction must be an active axis");
+    avh_status r = use_device(s);
+    if (r) return r;
+    AVH_HIP(hipStreamSynchronize(s->stream));
+    const size_t bytes = (size_t)s->nv * s->ncells * sizeof(double);
+    // the B and C buffers are rewritten by every step; use them as scratch
+    bool ok = false;
+    switch (s->nv) {
+    case 1: ok = launch_reconstruct_nv<1>(s->cfg.reconstruction, s->A, s->B, s->C, s->ncells, s->cfg.n, direction, s->cfg.boundary, s->stream); break;
+    case 3: ok = launch_reconstruct_nv<3>(s->cfg.reconstruction, s->A, s->B, s->C, s->ncells, s->cfg.n, direction, s->cfg.boundary, s->stream); break;
+    case 4: ok = launch_reconstruct_nv<4>(s->cfg.reconstruction, s->A, s->B, s->C, s->ncells, s->cfg.n, direction, s->cfg.boundary, s->stream); break;
+    case 5: ok = launch_reconstruct_nv<5>(s->cfg.reconstruction, s->A, s->B, s->C, s->ncells, s->cfg.n, direction, s->cfg.boundary, s->stream); break;
+    }
+    if (!ok) return fail(AVH_EINVAL, "unsupported reconstruction");
+    AVH_HIP(hipGetLastError());
+    AVH_HIP(hipMemcpyAsync(host_left, s->B, bytes, hipMemcpyDeviceToHost, s->stream));
+    AVH_HIP(hipMemcpyAsync(host_right, s->C, bytes, hipMemcpyDeviceToHost, s->stream));
+    AVH_HIP(hipStreamSynchronize(s->stream));
     return AVH_OK;
 }
 

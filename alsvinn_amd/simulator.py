@@ -241,6 +241,15 @@ class Simulator:
     def stream(self):
         return _abi.load().avh_stream(self._h)
 
+    def reconstruct(self, direction):
+        """(left, right) face values of every interior cell along `direction`
+        (Reconstruction::performReconstruction), each (nv, nz, ny, nx)."""
+        shape = (self.nv, self.n[2], self.n[1], self.n[0])
+        left = np.empty(shape)
+        right = np.empty(shape)
+        _abi.check(_abi.load().avh_reconstruct(self._h, int(direction), left.ctypes.data, right.ctypes.data))
+        return left, right
+
     def compute_flux(self):
         """NumericalFlux::computeFlux on the current state: unscaled net flux
         (interior layout) and the per-direction max wave speeds."""

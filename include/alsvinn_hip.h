@@ -185,6 +185,13 @@ void* avh_stream(avh_solver* solver);
  * writes the unscaled net-flux divergence (interior-only SoA) to
  * host_interior_out and the per-direction max wave speeds to ws[3]. */
 avh_status avh_compute_flux(avh_solver* solver, double* host_interior_out, double ws[3]);
+/* Unfused reconstruction of the current state along one active axis:
+ * left[v][cell] / right[v][cell] = the value at the cell's lower / upper
+ * face, interior SoA, ghosts through the boundary conditions
+ * (Reconstruction::performReconstruction,
+ * alsfvm/include/alsfvm/reconstruction/Reconstruction.hpp:46-58; the stage
+ * kernels fuse this step).  Synchronises. */
+avh_status avh_reconstruct(avh_solver* solver, int direction, double* host_left, double* host_right);
 
 /* ------------------------------------------------------------------
  * Batched Monte Carlo (alsuq on the hip platform).

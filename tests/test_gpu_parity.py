@@ -201,3 +201,27 @@ def test_burgers3d_several_z_chunks_bitwise():
     q0 = smooth_state("burgers", n, seed=29, jump=True)
     o, h = run_both(pyoracle, "burgers", "weno2", "godunov", "rungekutta3", "periodic", n, 0.3, q0, steps=3)
     np.testing.assert_array_equal(h.get_state(), o.get_state())
+
+
+@pytest.mark.parametrize("recon", ["none", "weno2", "eno2", "eno3"])
+@pytest.mark.parametrize("eq,n,bc", [("burgers", (37, 1, 1), "periodic"), ("euler2", (21, 18, 1), "neumann"),
+                                     ("euler3", (9, 10, 11), "periodic")])
+def test_unfused_reconstruction_bitwise(recon, eq, n, bc):
+    """avh_reconstruct (Reconstruction::performReconstruction) equals the
+    oracle's ReconstructionCPU / ENOCPU passes on the boundary-filled state,
+    cell for cell, every active direction."""
+    from oracle import pyoracle
+    dim = 3 if n[2] > 1 else (2 if n[1] > 1 else 1)
+    flux = "godunov" if eq == "burgers" else "hll3"
+    q = smooth_state(eq, n[:dim], seed=17, jump=True)
+    h = alsvinn_amd.Simulator(eq, recon, flux, "rungekutta3", bc, n)
+    h.set_interior(q)
+    o = pyoracle.Simulator(eq, recon, flux, "rungekutta3", bc, n)
+    o.set_interior(q)
+    U = o.get_state()
+    cfg = pyoracle.make_config(eq, recon, flux, "rungekutta3", bc, n)
+    for d in range(dim):
+        left, right = h.reconstruct(d)
+        lo, ro = pyoracle.reconstruct(cfg, U, d)
+        np.testing.assert_array_equal(left, h.interior(lo))
+        np.testing.assert_array_equal(right, h.interior(ro))
