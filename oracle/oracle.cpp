@@ -29,6 +29,14 @@
  * ghost index maps, grid midpoints), convergence rates, RK polynomials,
  * conservation under periodic BCs, Burgers characteristics before the
  * shock, exact Riemann solution for Sod, and interior+strips == whole domain.
+ *
+ * Observables (tests/test_oracle_observables.py): oracle_structure_cube
+ * follows computeStructureCubeCPU's loops and per-term division and is
+ * pinned by the reference's StructureTestBoundary cases (constant, linear
+ * 1D on N = 16, linear 2D on 16 x 16, p = 1..8, periodic and neumann) at the
+ * reference's own 4-ulp bar, plus a vectorised brute force;
+ * oracle_kinetic_energy (a new functional) by closed forms (uniform flow,
+ * sin^2 under the periodic midpoint rule).
  */
 #include <algorithm>
 #include <cmath>
