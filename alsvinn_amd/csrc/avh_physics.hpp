@@ -10,6 +10,40 @@
 
 namespace avh {
 
+// Correctly rounded FP64 square root.  For x in [2^-767, DBL_MAX] this is
+// the operation sequence the compiler emits for sqrt() on gfx950 -- v_rsq_f64,
+// then g = x*r, h = r/2 and two Goldschmidt / Newton corrections by fused
+// multiply-adds -- without the steps that are the identity on that range
+// (scaling tiny arguments by 2^256 and the result back by 2^-128, and the
+// zero / infinity fix-up), so the result has the same bits as sqrt(x).  Any
+// other argument (tiny, zero, negative, infinite, NaN) takes sqrt() itself.
+__device__ __forceinline__ double sqrt_rn(double x) {
+    if (x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023) {
+        const double r = __builtin_amdgcn_rsq(x);
+        double g = x * r;
+        double h = r * 0.5;
+        const double d0 = fma(-h, g, 0.5);
+        g = fma(g, d0, g);
+        h = fma(h, d0, h);
+        const double d1 = fma(-g, g, x);
+        g = fma(d1, h, g);
+        const double d2 = fma(-g, g, x);
+        return fma(d2, h, g);
+    }
+    return sqrt(x);
+}
+
+// sqrt_rn or sqrt(): the same bits either way; the 3D kernel, whose code is
+// large, keeps the compiler's single expansion.
+template <bool FAST>
+__device__ __forceinline__ double sqrt_sel(double x) {
+    if constexpr (FAST) {
+        return sqrt_rn(x);
+    } else {
+        return sqrt(x);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Equations.  A conserved state is NV doubles: burgers {u};
 // euler<nsd> {rho, m[0..nsd-1], E}.
@@ -90,6 +124,7 @@ struct EulerEq {
 // burgers::Godunov (alsfvm/include/alsfvm/numflux/burgers/Godunov.hpp:40-60):
 // F = max(f(max(uL,0)), f(min(uR,0))), wave speed max(|uL|, |uR|).
 struct GodunovFlux {
+    typedef GodunovFlux Plain;
     template <int DIR, class EQ>
     __device__ __forceinline__ static double flux(const BurgersEq::All& L, const BurgersEq::All& R,
                                                   double, double* F) {
@@ -103,20 +138,22 @@ struct GodunovFlux {
 // euler::HLL (alsfvm/include/alsfvm/numflux/euler/HLL.hpp).  The reference's
 // if/else chain is evaluated branch-free: every candidate is computed and the
 // chain's answer is selected, which gives the same value as the branch taken.
-struct HLLFlux {
+template <bool FAST_SQRT>
+struct HLLFluxT {
+    typedef HLLFluxT<false> Plain;
     template <int DIR, class EQ>
     __device__ __forceinline__ static double flux(const typename EQ::All& L, const typename EQ::All& R,
                                                   double g, double* F) {
         constexpr int NSD = EQ::NSD;
         constexpr int NV = EQ::NV;
-        const double wl = sqrt(L.rho);
-        const double wr = sqrt(R.rho);
+        const double wl = sqrt_sel<FAST_SQRT>(L.rho);
+        const double wr = sqrt_sel<FAST_SQRT>(R.rho);
         const double rho = (L.rho + R.rho) / 2;
         const double ud = (L.u[DIR] * wl + R.u[DIR] * wr) / (wl + wr);
         const double p = (L.p * wl + R.p * wr) / (wl + wr);
-        const double cs = sqrt(g * p / rho);
-        const double sl = fmin(L.u[DIR] - sqrt(g * L.p / L.rho), ud - cs);
-        const double sr = fmax(R.u[DIR] + sqrt(g * R.p / R.rho), ud + cs);
+        const double cs = sqrt_sel<FAST_SQRT>(g * p / rho);
+        const double sl = fmin(L.u[DIR] - sqrt_sel<FAST_SQRT>(g * L.p / L.rho), ud - cs);
+        const double sr = fmax(R.u[DIR] + sqrt_sel<FAST_SQRT>(g * R.p / R.rho), ud + cs);
         double fl[NV], fr[NV], ul[NV], ur[NV];
         EQ::template point_flux<DIR>(L, fl);
         EQ::template point_flux<DIR>(R, fr);
@@ -147,22 +184,24 @@ struct HLLFlux {
 // euler::HLL3 (alsfvm/include/alsfvm/numflux/euler/HLL3.hpp:52-148).  The two
 // star-state branches (sm >= 0 / sm < 0) share one formula on the side's
 // variables, so the side is selected first and the formula evaluated once.
-struct HLL3Flux {
+template <bool FAST_SQRT>
+struct HLL3FluxT {
+    typedef HLL3FluxT<false> Plain;
     template <int DIR, class EQ>
     __device__ __forceinline__ static double flux(const typename EQ::All& L, const typename EQ::All& R,
                                                   double g, double* F) {
         constexpr int NSD = EQ::NSD;
         constexpr int NV = EQ::NV;
         // computeHLLSpeeds
-        const double wl = sqrt(L.rho);
-        const double wr = sqrt(R.rho);
+        const double wl = sqrt_sel<FAST_SQRT>(L.rho);
+        const double wr = sqrt_sel<FAST_SQRT>(R.rho);
         const double rho = (L.rho + R.rho) / 2;
         const double ud = (L.u[DIR] * wl + R.u[DIR] * wr) / (wl + wr);
         const double p = (wl * L.p + wr * R.p) / (wl + wr);
-        const double cfl = sqrt(g * L.p / L.rho);
-        const double cfr = sqrt(g * R.p / R.rho);
+        const double cfl = sqrt_sel<FAST_SQRT>(g * L.p / L.rho);
+        const double cfr = sqrt_sel<FAST_SQRT>(g * R.p / R.rho);
         const double correct = 0.5 * fmax(0.0, L.u[DIR] - R.u[DIR]);
-        const double cs = sqrt(g * p / rho);
+        const double cs = sqrt_sel<FAST_SQRT>(g * p / rho);
         const double sl = fmin(L.u[DIR] + correct - cfl, ud - cs);
         const double sr = fmax(R.u[DIR] - correct + cfr, ud + cs);
 
@@ -218,6 +257,9 @@ struct HLL3Flux {
         return fmax(fabs(sl), fabs(sr));
     }
 };
+
+typedef HLLFluxT<true> HLLFlux;
+typedef HLL3FluxT<true> HLL3Flux;
 
 // ---------------------------------------------------------------------------
 // Reconstructions on an LDS tile.  `s` points at variable 0 of the tile,

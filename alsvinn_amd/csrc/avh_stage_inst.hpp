@@ -98,10 +98,11 @@ hipError_t launch_march_k(const StageParams& P, hipStream_t stream) {
 template <class EQ, int RECON, class FLUX, int DIM>
 hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
     if constexpr (DIM == 3) {
+        typedef typename FLUX::Plain F3;  // compiler sqrt: the 3D kernel's code is large
         switch (shape) {
-        case 0: return launch_march_k<MarchKernel3<EQ, RECON, FLUX, SHAPE3_TX[0], SHAPE3_TY[0], 32, 256>>(P, stream);
-        case 1: return launch_march_k<MarchKernel3<EQ, RECON, FLUX, SHAPE3_TX[1], SHAPE3_TY[1], 32, 256>>(P, stream);
-        case 2: return launch_march_k<MarchKernel3<EQ, RECON, FLUX, SHAPE3_TX[2], SHAPE3_TY[2], 32, 256>>(P, stream);
+        case 0: return launch_march_k<MarchKernel3<EQ, RECON, F3, SHAPE3_TX[0], SHAPE3_TY[0], 32, 256>>(P, stream);
+        case 1: return launch_march_k<MarchKernel3<EQ, RECON, F3, SHAPE3_TX[1], SHAPE3_TY[1], 32, 256>>(P, stream);
+        case 2: return launch_march_k<MarchKernel3<EQ, RECON, F3, SHAPE3_TX[2], SHAPE3_TY[2], 32, 256>>(P, stream);
         }
         return hipErrorInvalidValue;
     } else if constexpr (DIM == 1) {
