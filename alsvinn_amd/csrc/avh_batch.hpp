@@ -53,7 +53,7 @@ __device__ __forceinline__ int batch_group(const BatchParams& B, unsigned int b)
 }
 
 template <class K>
-__global__ __launch_bounds__(256) void batch_stage_kernel(BatchParams B) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void batch_stage_kernel(BatchParams B) {
     const unsigned int b = tile_id_xcd(blockIdx.x, gridDim.x);
     const int gi = batch_group(B, b);
     const GroupDesc& G = B.g[gi];
@@ -132,8 +132,7 @@ __device__ __forceinline__ int finish_sample(Control* c, int stats_adj) {
         }
     }
     c->block_violation = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) c->ws_bits[d] = 0ull;
+    clear_wave_speeds(c);
     c->active = (!(c->t >= c->t_end) && !c->violated) ? 1 : 0;
     return hit;
 }

@@ -57,8 +57,7 @@ __global__ __launch_bounds__(64) void finish_step_kernel(Control* c) {
         }
     }
     c->block_violation = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) c->ws_bits[d] = 0ull;
+    clear_wave_speeds(c);
     const bool more = !(c->t >= c->t_end) && !c->violated && !(c->pause_on_save && c->saves_pending > 0);
     c->active = more ? 1 : 0;
 }
@@ -919,12 +918,8 @@ avh_status avh_compute_flux(avh_solver* s, double* host_out, double ws[3]) {
     AVH_HIP(hipMemcpyAsync(host_out, s->B, (size_t)s->nv * s->ncells * sizeof(double), hipMemcpyDeviceToHost,
                            s->stream));
     if ((r = pull(s))) return r;
-    for (int d = 0; d < 3; ++d) {
-        double w;
-        std::memcpy(&w, &s->h.ws_bits[d], sizeof w);
-        ws[d] = w;
-        s->h.ws_bits[d] = 0;
-    }
+    for (int d = 0; d < 3; ++d) ws[d] = wave_speed(&s->h, d);
+    clear_wave_speeds(&s->h);
     return push(s);
 }
 

@@ -10,39 +10,83 @@
 
 namespace avh {
 
-// Correctly rounded FP64 square root.  For x in [2^-767, DBL_MAX] this is
-// the operation sequence the compiler emits for sqrt() on gfx950 -- v_rsq_f64,
+// Correctly rounded FP64 square root for x in [2^-767, DBL_MAX]: the
+// operation sequence the compiler emits for sqrt() on gfx950 -- v_rsq_f64,
 // then g = x*r, h = r/2 and two Goldschmidt / Newton corrections by fused
 // multiply-adds -- without the steps that are the identity on that range
 // (scaling tiny arguments by 2^256 and the result back by 2^-128, and the
-// zero / infinity fix-up), so the result has the same bits as sqrt(x).  Any
-// other argument (tiny, zero, negative, infinite, NaN) takes sqrt() itself.
-__device__ __forceinline__ double sqrt_rn(double x) {
-    if (x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023) {
-        const double r = __builtin_amdgcn_rsq(x);
-        double g = x * r;
-        double h = r * 0.5;
-        const double d0 = fma(-h, g, 0.5);
-        g = fma(g, d0, g);
-        h = fma(h, d0, h);
-        const double d1 = fma(-g, g, x);
-        g = fma(d1, h, g);
-        const double d2 = fma(-g, g, x);
-        return fma(d2, h, g);
-    }
-    return sqrt(x);
+// zero / infinity fix-up), so the result has the same bits as sqrt(x) there.
+__device__ __forceinline__ double sqrt_core(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r;
+    double h = r * 0.5;
+    const double d0 = fma(-h, g, 0.5);
+    g = fma(g, d0, g);
+    h = fma(h, d0, h);
+    const double d1 = fma(-g, g, x);
+    g = fma(d1, h, g);
+    const double d2 = fma(-g, g, x);
+    return fma(d2, h, g);
 }
 
-// sqrt_rn or sqrt(): the same bits either way; the 3D kernel, whose code is
-// large, keeps the compiler's single expansion.
-template <bool FAST>
-__device__ __forceinline__ double sqrt_sel(double x) {
-    if constexpr (FAST) {
-        return sqrt_rn(x);
-    } else {
-        return sqrt(x);
-    }
+// Quotients that share a denominator.  The compiler divides n/d in FP64 on
+// gfx950 as
+//   d' = div_scale(d), n' = div_scale(n), r = rcp(d'),
+//   r = fma(r, fma(-d', r, 1), r) twice, q0 = n' * r,
+//   q = div_fmas(fma(-d', q0, n'), r, q0), result div_fixup(q, d, n).
+// When d > 0, d and 1/d are normal, n is zero or has |n| >= 2^-968 and the
+// quotient's binary exponent lies in [-1000, 760], the scalings are the
+// identity (no VCC scale), div_fmas is a plain fma and div_fixup returns q
+// -- or, for n = +-0, the zero with the sign of n.  The quotient is then a
+// function of the refined reciprocal r alone, so r is computed once per
+// denominator (div_recip) and each numerator costs one multiply and two
+// fmas (div_shared).  The residual is formed as d*q0 - n and subtracted, the
+// exact negation of the compiler's n - d*q0 added: nonzero numerators give
+// the same bits, and n = +-0 gives +-0 as div_fixup does.
+__device__ __forceinline__ double div_recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return r;
 }
+__device__ __forceinline__ double div_shared(double n, double d, double r) {
+    const double q0 = n * r;
+    return fma(-fma(d, q0, -n), r, q0);
+}
+// n is zero or lo <= |n| <= hi (false for inf and NaN)
+__device__ __forceinline__ bool zero_or_within(double n, double lo, double hi) {
+    const double a = fabs(n);
+    return a <= hi && (a >= lo || n == 0.0);
+}
+
+// Arithmetic of an interface's reconstruction and flux.
+//   PlainOps   the compiler's sqrt and divide everywhere.
+//   SharedOps  sqrt_core, and shared reciprocals for the divisions by rho,
+//              by sqrt(rhoL) + sqrt(rhoR) and by the WENO weight sum; every
+//              operand is checked against the range where this gives the
+//              bits of PlainOps, and a failed check clears `ok`, after which
+//              the caller recomputes the interface with PlainOps.  The
+//              checks feed a flag instead of branching so that the interface
+//              stays one straight-line block for the scheduler.
+struct PlainOps {
+    static constexpr bool SHARED = false;
+    __device__ __forceinline__ static double sqrt_(double x, bool&) { return sqrt(x); }
+};
+struct SharedOps {
+    static constexpr bool SHARED = true;
+    __device__ __forceinline__ static double sqrt_(double x, bool& ok) {
+        ok = ok && x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023;
+        return sqrt_core(x);
+    }
+};
+template <bool SHARED>
+struct OpsFor {
+    typedef PlainOps type;
+};
+template <>
+struct OpsFor<true> {
+    typedef SharedOps type;
+};
 
 // ---------------------------------------------------------------------------
 // Equations.  A conserved state is NV doubles: burgers {u};
@@ -56,6 +100,8 @@ struct BurgersEq {
     };
     // makeAllVariables (alsfvm/include/alsfvm/equation/burgers/Burgers.hpp:220-224)
     __device__ __forceinline__ static All make_all(const double* q, double) { return All{q[0]}; }
+    template <class OPS>
+    __device__ __forceinline__ static All make_all_ops(const double* q, double g, bool&) { return make_all(q, g); }
     // computePointFlux: u*u/2 (Burgers.hpp:158-164)
     __device__ __forceinline__ static double point_flux(double u) { return u * u / 2; }
     // CPUCellComputer::obeysConstraints + Burgers::obeysConstraints
@@ -72,6 +118,7 @@ struct EulerEq {
     static constexpr int NV = NSD_ + 2;
     struct All {
         double rho, m[NSD_], E, p, u[NSD_];
+        double rinv;  // refined reciprocal of rho (div_recip), 0 when not formed
     };
     // vec::dot, left to right (alsutils/include/alsutils/vec{1,2,3}.hpp)
     __device__ __forceinline__ static double dot(const double* m) {
@@ -92,7 +139,64 @@ struct EulerEq {
 #pragma unroll
         for (int c = 0; c < NSD_; ++c) a.u[c] = a.m[c] / a.rho;
         a.p = (gamma - 1) * ie;
+        a.rinv = 0.0;
         return a;
+    }
+    // make_all under OPS.  SharedOps forms one reciprocal of rho for the
+    // divisions by rho; it checks that rho is in [2^-100, 2^100] and every
+    // momentum zero or in [2^-390, 2^290] in magnitude, so that the
+    // numerators 0.5*m.m and m[c] are zero or within [2^-781, 2^582] and
+    // every quotient within [2^-881, 2^682].
+    template <class OPS>
+    __device__ __forceinline__ static All make_all_ops(const double* q, double gamma, bool& ok) {
+        if constexpr (!OPS::SHARED) {
+            return make_all(q, gamma);
+        } else {
+            All a;
+            a.rho = q[0];
+#pragma unroll
+            for (int c = 0; c < NSD_; ++c) a.m[c] = q[1 + c];
+            a.E = q[NSD_ + 1];
+            ok = ok && a.rho >= 0x1p-100 && a.rho <= 0x1p100;
+#pragma unroll
+            for (int c = 0; c < NSD_; ++c) ok = ok && zero_or_within(a.m[c], 0x1p-390, 0x1p290);
+            const double r = div_recip(a.rho);
+            const double ie = a.E - div_shared(0.5 * dot(a.m), a.rho, r);
+#pragma unroll
+            for (int c = 0; c < NSD_; ++c) a.u[c] = div_shared(a.m[c], a.rho, r);
+            a.rinv = r;
+            a.p = (gamma - 1) * ie;
+            return a;
+        }
+    }
+    // n / rho of a state from make_all_ops<OPS>; SharedOps checks that n is
+    // zero or within [2^-800, 2^600] (quotient within [2^-900, 2^700])
+    template <class OPS>
+    __device__ __forceinline__ static double div_rho(double n, const All& a, bool& ok) {
+        if constexpr (!OPS::SHARED) {
+            return n / a.rho;
+        } else {
+            ok = ok && zero_or_within(n, 0x1p-800, 0x1p600);
+            return div_shared(n, a.rho, a.rinv);
+        }
+    }
+    // (n1, n2) / (wl + wr) with wl, wr the square roots of the two sides'
+    // densities.  Under SharedOps the densities passed make_all_ops' check,
+    // so wl + wr is in [2^-50, 2^51]; the numerators are checked to be zero
+    // or within [2^-800, 2^600].
+    template <class OPS>
+    __device__ __forceinline__ static void div_roe(double n1, double n2, double wl, double wr, double& q1,
+                                                   double& q2, bool& ok) {
+        if constexpr (!OPS::SHARED) {
+            q1 = n1 / (wl + wr);
+            q2 = n2 / (wl + wr);
+        } else {
+            ok = ok && zero_or_within(n1, 0x1p-800, 0x1p600) && zero_or_within(n2, 0x1p-800, 0x1p600);
+            const double den = wl + wr;
+            const double r = div_recip(den);
+            q1 = div_shared(n1, den, r);
+            q2 = div_shared(n2, den, r);
+        }
     }
     // computePointFlux along DIR (Euler.hpp:172-181)
     template <int DIR>
@@ -125,6 +229,7 @@ struct EulerEq {
 // F = max(f(max(uL,0)), f(min(uR,0))), wave speed max(|uL|, |uR|).
 struct GodunovFlux {
     typedef GodunovFlux Plain;
+    static constexpr bool FAST = false;
     template <int DIR, class EQ>
     __device__ __forceinline__ static double flux(const BurgersEq::All& L, const BurgersEq::All& R,
                                                   double, double* F) {
@@ -133,27 +238,41 @@ struct GodunovFlux {
         F[0] = fmax(fl, fr);
         return fmax(fabs(L.u), fabs(R.u));
     }
+    template <class OPS, int DIR, class EQ>
+    __device__ __forceinline__ static double flux_ops(const BurgersEq::All& L, const BurgersEq::All& R, double g,
+                                                      double* F, bool&) {
+        return flux<DIR, EQ>(L, R, g, F);
+    }
 };
 
 // euler::HLL (alsfvm/include/alsfvm/numflux/euler/HLL.hpp).  The reference's
 // if/else chain is evaluated branch-free: every candidate is computed and the
 // chain's answer is selected, which gives the same value as the branch taken.
-template <bool FAST_SQRT>
+template <bool FAST_>
 struct HLLFluxT {
     typedef HLLFluxT<false> Plain;
+    static constexpr bool FAST = FAST_;  // the stage kernel runs flux_ops<SharedOps> first
     template <int DIR, class EQ>
     __device__ __forceinline__ static double flux(const typename EQ::All& L, const typename EQ::All& R,
                                                   double g, double* F) {
+        bool ok = true;
+        return flux_ops<PlainOps, DIR, EQ>(L, R, g, F, ok);
+    }
+    template <class OPS, int DIR, class EQ>
+    __device__ __forceinline__ static double flux_ops(const typename EQ::All& L, const typename EQ::All& R,
+                                                      double g, double* F, bool& ok) {
         constexpr int NSD = EQ::NSD;
         constexpr int NV = EQ::NV;
-        const double wl = sqrt_sel<FAST_SQRT>(L.rho);
-        const double wr = sqrt_sel<FAST_SQRT>(R.rho);
+        const double wl = OPS::sqrt_(L.rho, ok);
+        const double wr = OPS::sqrt_(R.rho, ok);
         const double rho = (L.rho + R.rho) / 2;
-        const double ud = (L.u[DIR] * wl + R.u[DIR] * wr) / (wl + wr);
-        const double p = (L.p * wl + R.p * wr) / (wl + wr);
-        const double cs = sqrt_sel<FAST_SQRT>(g * p / rho);
-        const double sl = fmin(L.u[DIR] - sqrt_sel<FAST_SQRT>(g * L.p / L.rho), ud - cs);
-        const double sr = fmax(R.u[DIR] + sqrt_sel<FAST_SQRT>(g * R.p / R.rho), ud + cs);
+        double ud, p;
+        EQ::template div_roe<OPS>(L.u[DIR] * wl + R.u[DIR] * wr, L.p * wl + R.p * wr, wl, wr, ud, p, ok);
+        const double cs = OPS::sqrt_(g * p / rho, ok);
+        const double cfl = OPS::sqrt_(EQ::template div_rho<OPS>(g * L.p, L, ok), ok);
+        const double cfr = OPS::sqrt_(EQ::template div_rho<OPS>(g * R.p, R, ok), ok);
+        const double sl = fmin(L.u[DIR] - cfl, ud - cs);
+        const double sr = fmax(R.u[DIR] + cfr, ud + cs);
         double fl[NV], fr[NV], ul[NV], ur[NV];
         EQ::template point_flux<DIR>(L, fl);
         EQ::template point_flux<DIR>(R, fr);
@@ -184,24 +303,31 @@ struct HLLFluxT {
 // euler::HLL3 (alsfvm/include/alsfvm/numflux/euler/HLL3.hpp:52-148).  The two
 // star-state branches (sm >= 0 / sm < 0) share one formula on the side's
 // variables, so the side is selected first and the formula evaluated once.
-template <bool FAST_SQRT>
+template <bool FAST_>
 struct HLL3FluxT {
     typedef HLL3FluxT<false> Plain;
+    static constexpr bool FAST = FAST_;  // the stage kernel runs flux_ops<SharedOps> first
     template <int DIR, class EQ>
     __device__ __forceinline__ static double flux(const typename EQ::All& L, const typename EQ::All& R,
                                                   double g, double* F) {
+        bool ok = true;
+        return flux_ops<PlainOps, DIR, EQ>(L, R, g, F, ok);
+    }
+    template <class OPS, int DIR, class EQ>
+    __device__ __forceinline__ static double flux_ops(const typename EQ::All& L, const typename EQ::All& R,
+                                                      double g, double* F, bool& ok) {
         constexpr int NSD = EQ::NSD;
         constexpr int NV = EQ::NV;
         // computeHLLSpeeds
-        const double wl = sqrt_sel<FAST_SQRT>(L.rho);
-        const double wr = sqrt_sel<FAST_SQRT>(R.rho);
+        const double wl = OPS::sqrt_(L.rho, ok);
+        const double wr = OPS::sqrt_(R.rho, ok);
         const double rho = (L.rho + R.rho) / 2;
-        const double ud = (L.u[DIR] * wl + R.u[DIR] * wr) / (wl + wr);
-        const double p = (wl * L.p + wr * R.p) / (wl + wr);
-        const double cfl = sqrt_sel<FAST_SQRT>(g * L.p / L.rho);
-        const double cfr = sqrt_sel<FAST_SQRT>(g * R.p / R.rho);
+        double ud, p;
+        EQ::template div_roe<OPS>(L.u[DIR] * wl + R.u[DIR] * wr, wl * L.p + wr * R.p, wl, wr, ud, p, ok);
+        const double cfl = OPS::sqrt_(EQ::template div_rho<OPS>(g * L.p, L, ok), ok);
+        const double cfr = OPS::sqrt_(EQ::template div_rho<OPS>(g * R.p, R, ok), ok);
         const double correct = 0.5 * fmax(0.0, L.u[DIR] - R.u[DIR]);
-        const double cs = sqrt_sel<FAST_SQRT>(g * p / rho);
+        const double cs = OPS::sqrt_(g * p / rho, ok);
         const double sl = fmin(L.u[DIR] + correct - cfl, ud - cs);
         const double sr = fmax(R.u[DIR] - correct + cfr, ud + cs);
 
@@ -290,8 +416,26 @@ struct ReconTraits<RECON_ENO3> {
 // WENO2 (alsfvm/include/alsfvm/reconstruction/WENO2.hpp:31-95; epsilon 1e-8,
 // WENOCoefficients.hpp:21).  RIGHT selects the value at the cell's right face
 // (the reference's `right` volume), otherwise the left face.
-template <int NV, bool RIGHT>
-__device__ __forceinline__ void weno2_face(const double* s, int vs, int i, int sd, double* out) {
+// Under SharedOps the two weights share the reciprocal of a0 + a1, checked
+// to need a0, a1 >= 2^-400: both are at most 2/(3*eps^2) < 2^53, so the
+// numerators, the denominator and the quotients (within [2^-454, 1]) are in
+// the range where div_shared equals the division.
+template <class OPS>
+__device__ __forceinline__ void weno2_weights(double a0, double a1, double& w0, double& w1, bool& ok) {
+    if constexpr (OPS::SHARED) {
+        ok = ok && a0 >= 0x1p-400 && a1 >= 0x1p-400;
+        const double den = a0 + a1;
+        const double r = div_recip(den);
+        w0 = div_shared(a0, den, r);
+        w1 = div_shared(a1, den, r);
+    } else {
+        w0 = a0 / (a0 + a1);
+        w1 = a1 / (a0 + a1);
+    }
+}
+
+template <int NV, bool RIGHT, class OPS>
+__device__ __forceinline__ void weno2_face(const double* s, int vs, int i, int sd, double* out, bool& ok) {
     const double eps = 1e-8;
     const double w0 = s[i];
     const double dp = s[i + sd] - w0;
@@ -303,8 +447,8 @@ __device__ __forceinline__ void weno2_face(const double* s, int vs, int i, int s
     if (RIGHT) {
         const double a0 = 2 / d0;
         const double a1 = 1 / d1;
-        const double w0r = a0 / (a0 + a1);
-        const double w1r = a1 / (a0 + a1);
+        double w0r, w1r;
+        weno2_weights<OPS>(a0, a1, w0r, w1r, ok);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const double* u = s + v * vs;
@@ -313,8 +457,8 @@ __device__ __forceinline__ void weno2_face(const double* s, int vs, int i, int s
     } else {
         const double a0 = 1 / d0;
         const double a1 = 2 / d1;
-        const double w0l = a0 / (a0 + a1);
-        const double w1l = a1 / (a0 + a1);
+        double w0l, w1l;
+        weno2_weights<OPS>(a0, a1, w0l, w1l, ok);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const double* u = s + v * vs;
@@ -379,18 +523,24 @@ __device__ __forceinline__ void eno_face(const double* s, int vs, int i, int sd,
     }
 }
 
-template <int RECON, int NV, bool RIGHT>
-__device__ __forceinline__ void recon_face(const double* s, int vs, int i, int sd, double* out) {
+template <int RECON, int NV, bool RIGHT, class OPS>
+__device__ __forceinline__ void recon_face_ops(const double* s, int vs, int i, int sd, double* out, bool& ok) {
     if (RECON == RECON_NONE) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) out[v] = s[v * vs + i];
     } else if (RECON == RECON_WENO2) {
-        weno2_face<NV, RIGHT>(s, vs, i, sd, out);
+        weno2_face<NV, RIGHT, OPS>(s, vs, i, sd, out, ok);
     } else if (RECON == RECON_ENO2) {
         eno_face<2, NV, RIGHT>(s, vs, i, sd, out);
     } else {
         eno_face<3, NV, RIGHT>(s, vs, i, sd, out);
     }
+}
+
+template <int RECON, int NV, bool RIGHT>
+__device__ __forceinline__ void recon_face(const double* s, int vs, int i, int sd, double* out) {
+    bool ok = true;
+    recon_face_ops<RECON, NV, RIGHT, PlainOps>(s, vs, i, sd, out, ok);
 }
 
 }  // namespace avh

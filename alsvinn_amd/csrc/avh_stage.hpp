@@ -110,6 +110,17 @@ __device__ __forceinline__ unsigned int tile_id_xcd(unsigned int b, unsigned int
     return (b % 8u) * (nfull / 8u) + b / 8u;
 }
 
+// largest wave speed recorded along direction d
+__host__ __device__ __forceinline__ double wave_speed(const Control* c, int d) {
+    double w;
+    __builtin_memcpy(&w, &c->ws_bits[d], sizeof w);
+    return w;
+}
+
+__host__ __device__ __forceinline__ void clear_wave_speeds(Control* c) {
+    for (int d = 0; d < 3; ++d) c->ws_bits[d] = 0ull;
+}
+
 // dt of the step in flight: cfl / sum_d(ws_d / dx_d), then the
 // FixedIntervalWriter clips (alsfvm/src/integrator/Integrator.cpp:20-66,
 // alsfvm/src/io/FixedIntervalWriter.cpp:45-53).  ws_d are the maxima the
@@ -120,9 +131,8 @@ __device__ __forceinline__ double step_dt(const Control* c) {
     double total = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double ws = __longlong_as_double((long long)c->ws_bits[d]);
         if (c->cell[d] == 0) continue;
-        total += ws / c->cell[d];
+        total += wave_speed(c, d) / c->cell[d];
     }
     double dt = c->cfl / total;
     for (int i = 0; i < c->n_adj; ++i) {
@@ -160,47 +170,86 @@ struct StageKernel {
         return ((cz + HZ) * LY + (cy + HY)) * LX + (cx + HX);
     }
 
+    // reconstruct both sides of the interface between cell ia and ia + SD
+    // (LDS indices) and evaluate the numerical flux under OPS
+    template <class OPS, int DIR>
+    __device__ __forceinline__ static double interface(const double* su, int ia, int SD, double gamma,
+                                                       double* F, bool& ok) {
+        double qa[NV], qb[NV];
+        recon_face_ops<RECON, NV, true, OPS>(su, LT, ia, SD, qa, ok);        // right face of a
+        recon_face_ops<RECON, NV, false, OPS>(su, LT, ia + SD, SD, qb, ok);  // left face of b
+        const typename EQ::All aL = EQ::template make_all_ops<OPS>(qa, gamma, ok);
+        const typename EQ::All aR = EQ::template make_all_ops<OPS>(qb, gamma, ok);
+        return FLUX::template flux_ops<OPS, DIR, EQ>(aL, aR, gamma, F, ok);
+    }
+
+    // cell a = (cx, cy, cz) left of interface `it` along DIR, and whether the
+    // interface lies inside the grid
+    template <int DIR>
+    __device__ __forceinline__ static bool interface_cell(int it, int ox, int oy, int oz, const int* n, int& ia) {
+        int cx, cy, cz, k;
+        if (DIR == 0) {
+            k = it % (TX + 1);
+            const int r = it / (TX + 1);
+            cy = r % TY;
+            cz = r / TY;
+            cx = k - 1;
+        } else if (DIR == 1) {
+            cx = it % TX;
+            const int r = it / TX;
+            k = r % (TY + 1);
+            cz = r / (TY + 1);
+            cy = k - 1;
+        } else {
+            cx = it % TX;
+            cy = (it / TX) % TY;
+            k = it / (TX * TY);
+            cz = k - 1;
+        }
+        ia = lidx(cx, cy, cz);
+        const int ga = (DIR == 0 ? ox + cx : (DIR == 1 ? oy + cy : oz + cz));
+        return (ga < n[DIR]) && (oy + (DIR == 1 ? 0 : cy) < n[1]) && (ox + (DIR == 0 ? 0 : cx) < n[0]) &&
+               (oz + (DIR == 2 ? 0 : cz) < n[2]);
+    }
+
+    // Fluxes of every interface along DIR into sf.  A FAST flux evaluates each
+    // interface with shared reciprocals (SharedOps); the interfaces whose
+    // operands fall outside the range where that is exact are redone with
+    // PlainOps in a second loop, after the first one's registers are free.
     template <int DIR>
     __device__ __forceinline__ static void flux_pass(const double* su, double* sf, int ox, int oy, int oz,
                                                      const int* n, double gamma, double& ws) {
         constexpr int NI = DIR == 0 ? NI_X : (DIR == 1 ? NI_Y : NI_Z);
         constexpr int SD = DIR == 0 ? 1 : (DIR == 1 ? LX : LX * LY);
-        for (int it = threadIdx.x; it < NI; it += NT) {
-            int cx, cy, cz, k;
-            if (DIR == 0) {
-                k = it % (TX + 1);
-                const int r = it / (TX + 1);
-                cy = r % TY;
-                cz = r / TY;
-                cx = k - 1;
-            } else if (DIR == 1) {
-                cx = it % TX;
-                const int r = it / TX;
-                k = r % (TY + 1);
-                cz = r / (TY + 1);
-                cy = k - 1;
-            } else {
-                cx = it % TX;
-                cy = (it / TX) % TY;
-                k = it / (TX * TY);
-                cz = k - 1;
-            }
-            // interface between cell a = (cx,cy,cz) and the next cell along DIR
-            const int ga = (DIR == 0 ? ox + cx : (DIR == 1 ? oy + cy : oz + cz));
-            const bool need = (ga < n[DIR]) && (oy + (DIR == 1 ? 0 : cy) < n[1]) &&
-                              (ox + (DIR == 0 ? 0 : cx) < n[0]) && (oz + (DIR == 2 ? 0 : cz) < n[2]);
-            if (!need) continue;
-            const int ia = lidx(cx, cy, cz);
-            double qa[NV], qb[NV];
-            recon_face<RECON, NV, true>(su, LT, ia, SD, qa);        // right face of a
-            recon_face<RECON, NV, false>(su, LT, ia + SD, SD, qb);  // left face of b
-            const typename EQ::All aL = EQ::make_all(qa, gamma);
-            const typename EQ::All aR = EQ::make_all(qb, gamma);
+        static_assert((NI + NT - 1) / NT <= 32, "one redo bit per iteration");
+        typedef typename OpsFor<FLUX::FAST>::type Ops;
+        unsigned int redo = 0;
+        int m = 0;
+        for (int it = threadIdx.x; it < NI; it += NT, ++m) {
+            int ia;
+            if (!interface_cell<DIR>(it, ox, oy, oz, n, ia)) continue;
             double F[NV];
-            const double w = FLUX::template flux<DIR, EQ>(aL, aR, gamma, F);
-            ws = ws_max(ws, w);
+            bool ok = true;
+            const double w = interface<Ops, DIR>(su, ia, SD, gamma, F, ok);
+            if (ok) {
+                ws = ws_max(ws, w);
+            } else {
+                redo |= 1u << m;
+            }
 #pragma unroll
             for (int v = 0; v < NV; ++v) sf[v * NI_MAX + it] = F[v];
+        }
+        if (FLUX::FAST && redo != 0) {
+            m = 0;
+            for (int it = threadIdx.x; it < NI; it += NT, ++m) {
+                int ia;
+                if (!((redo >> m) & 1u) || !interface_cell<DIR>(it, ox, oy, oz, n, ia)) continue;
+                double F[NV];
+                bool ok = true;
+                ws = ws_max(ws, interface<PlainOps, DIR>(su, ia, SD, gamma, F, ok));
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sf[v * NI_MAX + it] = F[v];
+            }
         }
     }
 
@@ -254,8 +303,16 @@ struct StageKernel {
         const bool otf = (P.mode == MODE_RK3_2);
         const double s = (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) ? 0.0 : step_dt(ctrl) / ctrl->cell[0];
 
-        // 1. stage input tile + halo into LDS through the boundary map
-        for (int p = threadIdx.x; p < LT; p += NT) {
+        // 1. stage input tile + halo into LDS through the boundary map.  Each
+        // thread issues the loads of all its points before writing any of
+        // them, so that their global-memory latencies overlap; points past
+        // the tile repeat the last one and are not written.
+        constexpr int SPT = (LT + NT - 1) / NT;  // staged points per thread
+        long long sg[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            int p = threadIdx.x + k * NT;
+            p = p < LT ? p : LT - 1;
             const int lx = p % LX;
             const int ly = (p / LX) % LY;
             const int lz = p / (LX * LY);
@@ -266,13 +323,25 @@ struct StageKernel {
             gx = bc_map(gx, n0, P.bc);
             gy = DIM >= 2 ? bc_map(gy, n1, P.bc) : 0;
             gz = DIM >= 3 ? bc_map(gz, n2, P.bc) : 0;
-            const long long gi = ((long long)gz * n1 + gy) * n0 + gx;
+            sg[k] = ((long long)gz * n1 + gy) * n0 + gx;
+        }
+        double sv[SPT][NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                double val = P.in[v * nc + gi];
-                if (otf) val = val * s + P.u0[v * nc + gi];
-                su[v * LT + p] = val;
-            }
+        for (int k = 0; k < SPT; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sv[k][v] = P.in[v * nc + sg[k]];
+        if (otf) {
+#pragma unroll
+            for (int k = 0; k < SPT; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sv[k][v] = sv[k][v] * s + P.u0[v * nc + sg[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            const int p = threadIdx.x + k * NT;
+            if (LT % NT != 0 && p >= LT) continue;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) su[v * LT + p] = sv[k][v];
         }
         __syncthreads();
 
@@ -300,43 +369,57 @@ struct StageKernel {
             accumulate<DIM >= 3 ? 2 : 0>(sf, acc);
         }
 
-        // 4. RK combination and write-back
+        // 4. RK combination and write-back; the U0 loads of all the thread's
+        // cells are issued first (cells outside the grid load element 0 and
+        // write nothing)
         bool bad = false;
         const bool final_stage = (P.mode == MODE_RK2_2 || P.mode == MODE_RK3_3);
+        const bool combine = !(P.mode == MODE_FLUX_ONLY || P.mode == MODE_L);
+        long long gc[CPT];
+        bool inside[CPT];
 #pragma unroll
         for (int m = 0; m < CPT; ++m) {
             const int c = threadIdx.x + m * NT;
-            if (TC % NT != 0 && c >= TC) continue;
             const int cx = c % TX;
             const int cy = (c / TX) % TY;
             const int cz = c / (TX * TY);
             const int gx = ox + cx, gy = oy + cy, gz = oz + cz;
-            if (gx >= n0 || gy >= n1 || gz >= n2) continue;
-            const long long gi = ((long long)gz * n1 + gy) * n0 + gx;
+            inside[m] = (TC % NT == 0 || c < TC) && gx < n0 && gy < n1 && gz < n2;
+            gc[m] = inside[m] ? ((long long)gz * n1 + gy) * n0 + gx : 0;
+        }
+        const double ca = P.mode == MODE_RK3_2 ? 1. / 3. : (P.mode == MODE_RK3_3 ? 2.0 : 1.0);
+        const double cb = P.mode == MODE_RK2_2 ? 0.5 : (P.mode == MODE_RK3_2 ? 3. / 4. : 1. / 3.);
+        double u0c[CPT][NV];
+        if (combine) {
+#pragma unroll
+            for (int m = 0; m < CPT; ++m)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) u0c[m][v] = P.u0[v * nc + gc[m]];
+        }
+#pragma unroll
+        for (int m = 0; m < CPT; ++m) {
+            if (!inside[m]) continue;
+            const int c = threadIdx.x + m * NT;
+            const int cx = c % TX;
+            const int cy = (c / TX) % TY;
+            const int cz = c / (TX * TY);
+            const long long gi = gc[m];
             const int li = lidx(cx, cy, cz);
             double r[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const double l = acc[m][v];
-                if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
+                if (!combine) {
                     r[v] = l;
                 } else {
+                    // ((L*s + U) * ca + U0) * cb; the factor 1 of the RK2
+                    // stage is exact, so every mode keeps its own rounding
                     const double us = su[v * LT + li];
-                    const double u0 = P.u0[v * nc + gi];
                     double o = l * s;
                     o = o + us;
-                    if (P.mode == MODE_RK2_2) {
-                        o = o + u0;
-                        o = o * 0.5;
-                    } else if (P.mode == MODE_RK3_2) {
-                        o = o * (1. / 3.);
-                        o = o + u0;
-                        o = o * (3. / 4.);
-                    } else {
-                        o = o * 2;
-                        o = o + u0;
-                        o = o * (1. / 3.);
-                    }
+                    o = o * ca;
+                    o = o + u0c[m][v];
+                    o = o * cb;
                     r[v] = o;
                 }
             }
@@ -369,7 +452,7 @@ struct StageKernel {
 };
 
 template <class K>
-__global__ __launch_bounds__(256) void stage_kernel(StageParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void stage_kernel(StageParams P) {
     K::run(P);
 }
 

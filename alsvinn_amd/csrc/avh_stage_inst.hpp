@@ -18,20 +18,20 @@ typedef hipError_t (*BatchLauncher)(const BatchParams& B, unsigned int nblocks, 
 // order, so the choice never changes a result bit.
 //   1D: (cells, threads) = (256, 256) or (64, 64): small grids get more
 //       workgroups from the smaller tile.
-//   2D: 32x16 (LDS 40.8 KB, 4 waves/SIMD for euler2), 32x8 (23 KB, 5
-//       waves/SIMD), 64x4, 32x15 or 24x20 cells, 256 threads.  A flux pass
-//       hands interface items to threads in turns of 256, so a pass with a
-//       few items past a multiple of 256 gives wave 0 -- which sits on SIMD 0
-//       in every workgroup -- an extra turn: 32x16 has 528 x- and 544
-//       y-interfaces.  32x15 (495 / 512) and 24x20 (500 / 504) take exactly
-//       two turns on every wave.
+//   2D: 32x16, 16x15, 64x4, 32x15 or 24x20 cells, 256 threads (LDS 40.8,
+//       20.7, 28.0, 38.6 and 38.0 KB for euler2 with WENO2).  A
+//       flux pass hands interface items to threads in turns of 256, and a
+//       turn costs a wave its full instruction stream however few of its
+//       lanes have an item, so shapes whose interface counts sit just under
+//       a multiple of 256 waste the least: 32x15 (495 x- / 512 y-interfaces),
+//       24x20 (500 / 504) and 16x15 (255 / 256), against 32x16 (528 / 544).
 //   3D: columns marching over 32 planes, 256 threads: 32x8 (264 / 288
 //       plane interfaces), 16x15 or 32x7 (at most 256 each: one turn; the
 //       threads past the columns only work on the planes).
 constexpr int NSHAPES1 = 2, NSHAPES2 = 5, NSHAPES3 = 3;
 constexpr int SHAPE1_TX[NSHAPES1] = {256, 64};
-constexpr int SHAPE2_TX[NSHAPES2] = {32, 32, 64, 32, 24};
-constexpr int SHAPE2_TY[NSHAPES2] = {16, 8, 4, 15, 20};
+constexpr int SHAPE2_TX[NSHAPES2] = {32, 16, 64, 32, 24};
+constexpr int SHAPE2_TY[NSHAPES2] = {16, 15, 4, 15, 20};
 constexpr int SHAPE3_TX[NSHAPES3] = {32, 16, 32};
 constexpr int SHAPE3_TY[NSHAPES3] = {8, 15, 7};
 
