@@ -283,6 +283,21 @@ struct MarchKernel3 {
     }
         AVH_PREFETCH_RING(z0)
 
+        // RK combination ((L*s + U) * ca + U0) * cb (the factor 1 of the RK2
+        // stage is exact); U0 of the column's next plane is loaded one plane
+        // ahead, with the window's next plane
+        const bool combine = !(P.mode == MODE_FLUX_ONLY || P.mode == MODE_L);
+        const double ca = P.mode == MODE_RK3_2 ? 1. / 3. : (P.mode == MODE_RK3_3 ? 2.0 : 1.0);
+        const double cb = P.mode == MODE_RK2_2 ? 0.5 : (P.mode == MODE_RK3_2 ? 3. / 4. : 1. / 3.);
+        double u0n[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) u0n[v] = 0.0;
+        if (combine && col_ok) {
+            const long long gi = gidx(ox + cx, oy + cy, z0, n0, n1);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) u0n[v] = P.u0[v * P.ncells + gi];
+        }
+
         for (int k = z0; k < z1; ++k) {
             // plane k into LDS: own column from the window (w[HW-1] is plane k),
             // the halo ring from the prefetched registers
@@ -346,25 +361,14 @@ struct MarchKernel3 {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
                     const double l = acc[v];
-                    if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
+                    if (!combine) {
                         r[v] = l;
                     } else {
-                        const double us = w[HW - 1][v];
-                        const double u0 = P.u0[v * P.ncells + gi];
                         double o = l * s;
-                        o = o + us;
-                        if (P.mode == MODE_RK2_2) {
-                            o = o + u0;
-                            o = o * 0.5;
-                        } else if (P.mode == MODE_RK3_2) {
-                            o = o * (1. / 3.);
-                            o = o + u0;
-                            o = o * (3. / 4.);
-                        } else {
-                            o = o * 2;
-                            o = o + u0;
-                            o = o * (1. / 3.);
-                        }
+                        o = o + w[HW - 1][v];
+                        o = o * ca;
+                        o = o + u0n[v];
+                        o = o * cb;
                         r[v] = o;
                     }
                 }
@@ -382,7 +386,28 @@ struct MarchKernel3 {
             for (int i = 0; i < WZ - 1; ++i)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
-            if (has_col && k + 1 < z1) fetch(P, otf, s, gidx(gxc, gyc, bc_map(k + 2 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
+            // the window's next plane and the next write-back's U0, issued
+            // together so that their latencies overlap
+            if (has_col && k + 1 < z1) {
+                const long long gn = gidx(gxc, gyc, bc_map(k + 2 + HW, n2, P.bc), n0, n1);
+                const long long gw = gidx(ox + cx, oy + cy, k + 1, n0, n1);
+                double cin[NV], cu0[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    cin[v] = P.in[v * P.ncells + gn];
+                    cu0[v] = 0.0;
+                }
+                if (otf) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) cu0[v] = P.u0[v * P.ncells + gn];
+                }
+                if (combine && col_ok) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) u0n[v] = P.u0[v * P.ncells + gw];
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) w[WZ - 1][v] = otf ? cin[v] * s + cu0[v] : cin[v];
+            }
             __syncthreads();
         }
 

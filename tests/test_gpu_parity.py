@@ -113,6 +113,50 @@ def test_compute_flux_alternating_states_bitwise():
         np.testing.assert_array_equal(out_h, o.interior(out_o))
 
 
+def _extreme_state(equation, n, seed=7):
+    """A smooth euler state with cells whose operands leave the range of the
+    shared-reciprocal quotients: densities 1e-40 and 1e35, momenta 1e-130
+    (nonzero below 2^-390), exact +0 and -0 momenta, and density jumps that
+    drive a WENO2 weight below 2^-400."""
+    q = smooth_state(equation, n, seed=seed)
+    nsd = q.shape[0] - 2
+    flat = q.reshape(q.shape[0], -1)
+    ncell = flat.shape[1]
+    rng = np.random.default_rng(seed)
+    picks = rng.permutation(ncell)
+    k = max(1, ncell // 12)
+    tiny_rho, huge_rho, tiny_m, zero_m = np.split(picks[:4 * k], 4)
+    flat[0, tiny_rho] = 1e-40
+    flat[1:1 + nsd, tiny_rho] = 0.0
+    flat[-1, tiny_rho] = 1e-40
+    flat[0, huge_rho] = 1e35
+    flat[-1, huge_rho] = flat[-1, huge_rho] * 1e35
+    flat[1, tiny_m] = 1e-130 * np.where(rng.random(k) < 0.5, -1.0, 1.0)
+    flat[1:1 + nsd, zero_m] = np.where(rng.random((nsd, k)) < 0.5, -0.0, 0.0)
+    return flat.reshape(q.shape)
+
+
+@pytest.mark.parametrize("equation,n", [("euler2", (40, 20)), ("euler1", (70,))])
+@pytest.mark.parametrize("recon", ["weno2", "none"])
+@pytest.mark.parametrize("flux", ["hll", "hll3"])
+def test_shared_reciprocal_guards_bitwise(equation, n, recon, flux):
+    """Interfaces whose operands leave the range of the shared-reciprocal
+    quotients are redone with plain divisions: net flux and wave speeds keep
+    every bit of the oracle's, signed zeros included."""
+    from oracle import pyoracle
+    _say("shared-reciprocal guards", equation, n, recon, flux)
+    q0 = _extreme_state(equation, n)
+    o, h = make_pair(pyoracle, equation, recon, flux, "forwardeuler", "periodic", n, 0.4)
+    o.set_interior(q0)
+    h.set_interior(q0)
+    U = o.get_state()
+    out_o, ws_o = pyoracle.compute_flux(o.cfg, U, np.zeros_like(U))
+    out_h, ws_h = h.compute_flux()
+    assert np.array_equal(np.asarray(ws_h).view(np.uint64), np.asarray(ws_o).view(np.uint64))
+    ref = np.ascontiguousarray(o.interior(out_o))
+    assert np.array_equal(np.ascontiguousarray(out_h).view(np.uint64), ref.view(np.uint64))
+
+
 GRIDS = {
     "burgers": [(70,), (24, 20), (10, 9, 12)],
     "euler1": [(70,)],
