@@ -628,6 +628,14 @@ avh_status avh_enqueue_steps(avh_solver* s, int64_t steps) {
     return enqueue_steps(s, steps);
 }
 
+avh_status avh_prepare_graph(avh_solver* s) {
+    if (!s) return fail(AVH_EINVAL, "null argument");
+    avh_status r = use_device(s);
+    if (r) return r;
+    if ((r = ensure_shape(s))) return r;
+    return ensure_step_graph(s->graph, s->graph_steps, s->stream, [s]() { return enqueue_step(s); });
+}
+
 avh_status avh_set_graph_steps(avh_solver* s, int steps) {
     if (!s || steps < 0 || steps > 4096) return fail(AVH_EINVAL, "bad argument");
     avh_status r = use_device(s);

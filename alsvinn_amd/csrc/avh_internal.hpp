@@ -130,31 +130,39 @@ struct StepGraph {
 // Enqueues n steps: whole chunks of graph_steps as graph launches (captured
 // on first use), the remainder as ordinary launches. graph_steps < 2 turns
 // graphs off.
+// Captures and instantiates the chunk graph if it is not there yet; nothing
+// is launched.
+template <class Step>
+avh_status ensure_step_graph(StepGraph& G, int graph_steps, hipStream_t st, Step&& step) {
+    if (graph_steps < 2 || (G.steps == graph_steps && G.exec)) return AVH_OK;
+    G.reset();
+    AVH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    avh_status r = AVH_OK;
+    for (int i = 0; i < graph_steps && r == AVH_OK; ++i) r = step();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (r != AVH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(AVH_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e2 != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(AVH_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    }
+    G.graph = g;
+    G.exec = x;
+    G.steps = graph_steps;
+    return AVH_OK;
+}
+
 template <class Step>
 avh_status enqueue_steps_graphed(StepGraph& G, int graph_steps, hipStream_t st, int64_t n, Step&& step) {
     if (graph_steps >= 2 && n >= graph_steps) {
-        if (G.steps != graph_steps || !G.exec) {
-            G.reset();
-            AVH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            avh_status r = AVH_OK;
-            for (int i = 0; i < graph_steps && r == AVH_OK; ++i) r = step();
-            hipGraph_t g = nullptr;
-            const hipError_t e = hipStreamEndCapture(st, &g);
-            if (r != AVH_OK) {
-                if (g) (void)hipGraphDestroy(g);
-                return r;
-            }
-            if (e != hipSuccess) return fail(AVH_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-            hipGraphExec_t x = nullptr;
-            const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-            if (e2 != hipSuccess) {
-                (void)hipGraphDestroy(g);
-                return fail(AVH_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
-            }
-            G.graph = g;
-            G.exec = x;
-            G.steps = graph_steps;
-        }
+        avh_status rg = ensure_step_graph(G, graph_steps, st, step);
+        if (rg != AVH_OK) return rg;
         while (n >= graph_steps) {
             AVH_HIP(hipGraphLaunch(G.exec, st));
             n -= graph_steps;

@@ -238,6 +238,11 @@ class Simulator:
         """Steps per captured hipGraph (0: ordinary launches)."""
         _abi.check(_abi.load().avh_set_graph_steps(self._h, int(steps)))
 
+    def prepare_graph(self):
+        """Resolve the tile shape and capture the step-chunk graph now, so
+        that later steps only replay it."""
+        _abi.check(_abi.load().avh_prepare_graph(self._h))
+
     def stream(self):
         return _abi.load().avh_stream(self._h)
 

@@ -1,11 +1,17 @@
 """Benchmark of the hip platform on one MI355X.
 
-Default workload: 2D compressible Euler (gamma 1.4) Kelvin-Helmholtz on the
-unit square, 1024 x 1024 cells, periodic, FP64, HLL3 flux + WENO2 + RK3-TVD,
-CFL 0.4, seeded band perturbations (--workload kh3d: the 3D 256^3 case).
+Default workload: 3D compressible Euler (gamma 1.4) Kelvin-Helmholtz on the
+unit cube, 256^3 cells, periodic, FP64, HLL3 flux + WENO2 + RK3-TVD, CFL 0.3,
+seeded band perturbations over (x, z) -- the case whose working set (5
+variables x 3 buffers x 671 MB) is far beyond the 256 MB last-level cache,
+so the fused stage kernel's bytes come from HBM.  --workload kh2d runs the
+2D 1024^2 case (its 3 x 33.5 MB working set stays cache-resident, flagged in
+the output), --workload uq the batched Monte Carlo set.
 One step = one full RK3 time step (3 fused stage launches + on-device CFL
 dt) over the grid; the metric is FP64 cell-updates/s = cells * steps / wall
-time, the state resident in HBM before the timed region.
+time, the state resident in HBM before the timed region.  The tile shape is
+autotuned and the step-chunk hipGraph captured before the warmup steps, so
+the timed steps only replay.
 
 Prints one JSON line with the roofline of the fused stage kernel (HIP event
 timing on the solver's stream) and the CPU baseline (the oracle -- a
@@ -13,6 +19,7 @@ reference-structure restatement of the reference CPU path -- timed on a
 bounded number of steps of the same workload on the host cores).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -24,14 +31,60 @@ SHAPE_NAMES = {1: ["256", "64"], 2: ["32x16", "16x15", "64x4", "32x15", "24x20"]
 
 WORKLOADS = {
     # the bench line
-    "kh2d": dict(equation="euler2", n=(1024, 1024, 1), nv=4, cfl=0.4, profile="kh2d_1024",
+    "kh2d": dict(equation="euler2", n=(1024, 1024, 1), nv=4, cfl=0.4, profile="kh2d_1024", mall_resident=True,
                  desc="2D compressible Euler Kelvin-Helmholtz 1024x1024, periodic, HLL3 + WENO2 + RK3-TVD, "
                       "CFL 0.40, gamma 1.4, FP64"),
-    # 3D KH, working set beyond the 256 MB MALL
-    "kh3d": dict(equation="euler3", n=(256, 256, 256), nv=5, cfl=0.3, profile="kh3d_256",
+    # 3D KH: working set beyond the 256 MB last-level cache (the %HBM case)
+    "kh3d": dict(equation="euler3", n=(256, 256, 256), nv=5, cfl=0.3, profile="kh3d_256", mall_resident=False,
                  desc="3D compressible Euler Kelvin-Helmholtz 256^3, periodic, HLL3 + WENO2 + RK3-TVD, "
                       "CFL 0.30, gamma 1.4, FP64"),
 }
+
+# Sources of the stage kernels: a profile summary under profiles/ records the
+# digest of the sources it was measured on, and its traffic / instruction
+# figures are reported only while the sources still match.
+KERNEL_SOURCES = ["avh_physics.hpp", "avh_stage.hpp", "avh_stage3d.hpp", "avh_stage_inst.hpp", "avh_batch.hpp",
+                  "avh_stage_burgers.hip", "avh_stage_euler1.hip", "avh_stage_euler2.hip", "avh_stage_euler3.hip"]
+
+
+def kernel_source_digest():
+    h = hashlib.sha256()
+    csrc = os.path.join(os.path.dirname(os.path.abspath(__file__)), "alsvinn_amd", "csrc")
+    for name in KERNEL_SOURCES:
+        h.update(name.encode())
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def load_profile(name):
+    """The profile summary `name` if it was measured on the current kernel
+    sources, else None (and the reason)."""
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    if not os.path.exists(tfile):
+        return None, "no profile summary %s" % name
+    prof = json.load(open(tfile))
+    if prof.get("kernel_source_digest") != kernel_source_digest():
+        return None, "%s was measured on other kernel sources (digest %s, now %s)" % (
+            name, prof.get("kernel_source_digest"), kernel_source_digest())
+    return prof, None
+
+
+def host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "affinity": allowed}
 
 
 # Batched Monte Carlo of the 2D KH problem: 1024 samples at 256^2, 256 at
@@ -61,9 +114,13 @@ def initial_state(wl):
 
 
 def cpu_baseline(wl, seconds_budget=15.0):
+    # The host cores this process may use: OMP_NUM_THREADS when the
+    # environment sets it (the CPU share allotted beside one GPU), else every
+    # core of the affinity mask.
+    cpu = host_cpu()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     if threads <= 0:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu["affinity"] or 1
         os.environ["OMP_NUM_THREADS"] = str(threads)
     from oracle import pyoracle
 
@@ -86,6 +143,7 @@ def cpu_baseline(wl, seconds_budget=15.0):
         "kind": "port",
         "sample": "%d RK3 steps of the same workload (%s) on the reference-structure CPU restatement "
                   "(OpenMP, -O3 -ffp-contract=off), %.1f s" % (steps, "x".join(map(str, wl["n"])), el),
+        "host": dict(cpu, omp_num_threads=threads),
     }
 
 
@@ -162,11 +220,8 @@ def run_uq(args):
     avg_ms = sum(stage_ms) / 3
     avg_bytes = cells * sum(sb) / len(sb)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                         "batch_stage_kernel_uq_traffic.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    prof, stale = load_profile("batch_stage_kernel_uq_traffic.json")
+    traffic = prof.get("bytes_per_launch") if prof else None
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
@@ -202,6 +257,7 @@ def run_uq(args):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_note": stale,
             "kernel": "avh::batch_stage_kernel (fused reconstruct + flux + RK stage, all samples)",
             "algorithmic_bytes_per_launch": avg_bytes,
             "stage_ms": stage_ms,
@@ -222,7 +278,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=100)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["uq"], default="kh2d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["uq"], default="kh3d")
     ap.add_argument("--tile-shape", type=int, default=-1,
                     help="fix the stage kernel's tile shape (index into the autotuner's list) instead of "
                          "autotuning, e.g. to profile one kernel instance")
@@ -251,9 +307,9 @@ def main():
         sys.exit(2)
 
     if args.steps is None:
-        args.steps = 100 if args.workload == "uq" else 600
+        args.steps = {"uq": 100, "kh3d": 100}.get(args.workload, 600)
     if args.warmup is None:
-        args.warmup = 5 if args.workload == "uq" else 30
+        args.warmup = 5 if args.workload == "uq" else 10
     if args.workload == "uq":
         print(json.dumps(run_uq(args)), flush=True)
         return
@@ -272,6 +328,7 @@ def main():
     else:
         best, tune_ms = sim.autotune()
         tune = {"chosen": SHAPE_NAMES[dim][best], "ms_per_shape": dict(zip(SHAPE_NAMES[dim], tune_ms))}
+    sim.prepare_graph()  # capture + instantiate now: the timed steps only replay
 
     sim.enqueue_steps(args.warmup)
     sim.synchronize()
@@ -304,17 +361,16 @@ def main():
 
     traffic = None
     valu = None
-    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                         "stage_kernel_%s_traffic.json" % WORKLOADS[args.workload]["profile"])
-    if os.path.exists(tfile):
-        prof = json.load(open(tfile))
+    pname = "stage_kernel_%s_traffic.json" % WORKLOADS[args.workload]["profile"]
+    prof, stale = load_profile(pname)
+    if prof:
         traffic = prof.get("bytes_per_launch")
         insts = prof.get("valu_insts_per_launch")
         if insts:
             # second bound: FP64 VALU issue, 1024 SIMDs x one wave64 instruction per 4 cycles at 2.4 GHz
             peak = 1024 * 2.4e9 / 4
             valu = {"insts_per_launch": insts, "achieved_per_s": insts / (avg_ms * 1e-3), "peak_per_s": peak,
-                    "frac": insts / (avg_ms * 1e-3) / peak, "source": os.path.basename(tfile)}
+                    "frac": insts / (avg_ms * 1e-3) / peak, "source": pname}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -347,7 +403,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "avh::stage_kernel (fused reconstruct + flux + RK stage)",
+            "traffic_note": stale,
+            "mall_resident": wl["mall_resident"],
+            "kernel": ("avh::march_kernel3 (fused reconstruct + flux + RK stage, z-marching columns)"
+                       if dim == 3 else "avh::stage_kernel (fused reconstruct + flux + RK stage)"),
+            "kernel_source_digest": kernel_source_digest(),
             "algorithmic_bytes_per_launch": avg_bytes,
             "stage_ms": stage_ms,
             "end_of_step_kernel_ms": finish_ms,

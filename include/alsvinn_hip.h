@@ -156,6 +156,11 @@ avh_status avh_synchronize(avh_solver* solver);
  * once and replayed; results are identical either way.  The reference
  * launches each kernel separately (alsfvm/src/simulator/Simulator.cpp:191-214). */
 avh_status avh_set_graph_steps(avh_solver* solver, int steps);
+/* Resolve the tile shape (autotuning if none is fixed) and capture and
+ * instantiate the step-chunk graph now, without launching it, so that a
+ * later avh_enqueue_steps / avh_advance only replays.  No reference
+ * counterpart (set-up cost kept out of a timed or latency-bound loop). */
+avh_status avh_prepare_graph(avh_solver* solver);
 /* Tile shape of the fused stage kernel: -1 (default) lets the autotuner
  * pick on first use (cached per equation, reconstruction, flux and grid for
  * the process), 0..nshapes-1 fixes one (1D: 256 or 64 cells; 2D: 32x16,

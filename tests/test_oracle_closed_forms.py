@@ -131,3 +131,131 @@ def test_hll3_keeps_stationary_contact(oracle):
             s.step()
         err = np.abs(s.get_interior() - q).max()
         assert (err < 1e-13) == keeps, (flux, err)
+
+
+# --------------------------------------------------------------------------
+# HLL3 star states (alsfvm/include/alsfvm/numflux/euler/HLL3.hpp:52-148)
+
+GAMMA = 1.4
+EQ_OF = {1: "euler1", 2: "euler2", 3: "euler3"}
+
+
+def conserved(rho, u, p):
+    u = np.asarray(u, dtype=float)
+    return np.concatenate([[rho], rho * u, [p / (GAMMA - 1) + 0.5 * rho * np.dot(u, u)]])
+
+
+def primitive(q, nsd):
+    rho, m, E = q[0], q[1:1 + nsd], q[1 + nsd]
+    return rho, m / rho, (GAMMA - 1) * (E - 0.5 * np.dot(m, m) / rho), E
+
+
+def euler_flux(q, nsd, d):
+    rho, u, p, E = primitive(q, nsd)
+    F = np.empty(nsd + 2)
+    F[0] = rho * u[d]
+    F[1:1 + nsd] = rho * u * u[d]
+    F[1 + d] += p
+    F[1 + nsd] = (E + p) * u[d]
+    return F
+
+
+def random_pair(rng, nsd, mach=0.6):
+    """Two states with densities, pressures and velocities of order one."""
+    out = []
+    for _ in range(2):
+        rho = 0.3 + 2.0 * rng.random()
+        p = 0.3 + 2.0 * rng.random()
+        u = mach * (2 * rng.random(nsd) - 1)
+        out.append(conserved(rho, u, p))
+    return out
+
+
+def hllc_toro(qL, qR, nsd, d):
+    """The HLLC flux in the form of Toro, Riemann Solvers and Numerical Methods
+    for Fluid Dynamics (3rd ed.), eqs. (10.37)-(10.39): contact speed S*, star
+    states U*_K = rho_K (S_K - u_K)/(S_K - S*) (1, S*, v, E_K/rho_K +
+    (S* - u_K)(S* + p_K/(rho_K (S_K - u_K)))) and F*_K = F_K + S_K (U*_K - U_K).
+    The transverse velocity v of both star states is that of the HLL
+    intermediate state (Harten, Lax and van Leer 1983), U_hll = (S_R U_R -
+    S_L U_L - (F_R - F_L)) / (S_R - S_L).  The wave speeds are HLL3's: Roe
+    (sqrt(rho)) averages of u and p, the mean density, the per-side sound
+    speeds and the compression correction 0.5 max(0, u_L - u_R)."""
+    rL, uL, pL, EL = primitive(qL, nsd)
+    rR, uR, pR, ER = primitive(qR, nsd)
+    wl, wr = np.sqrt(rL), np.sqrt(rR)
+    ub = (wl * uL[d] + wr * uR[d]) / (wl + wr)
+    pb = (wl * pL + wr * pR) / (wl + wr)
+    rb = 0.5 * (rL + rR)
+    cL, cR, cb = np.sqrt(GAMMA * pL / rL), np.sqrt(GAMMA * pR / rR), np.sqrt(GAMMA * pb / rb)
+    corr = 0.5 * max(0.0, uL[d] - uR[d])
+    SL = min(uL[d] + corr - cL, ub - cb)
+    SR = max(uR[d] - corr + cR, ub + cb)
+    FL, FR = euler_flux(qL, nsd, d), euler_flux(qR, nsd, d)
+    if SL > 0:
+        return FL, max(abs(SL), abs(SR)), "left"
+    if SR < 0:
+        return FR, max(abs(SL), abs(SR)), "right"
+    Ss = (pR - pL + rL * uL[d] * (SL - uL[d]) - rR * uR[d] * (SR - uR[d])) / (rL * (SL - uL[d]) - rR * (SR - uR[d]))
+    Uhll = (SR * qR - SL * qL - (FR - FL)) / (SR - SL)
+    v = Uhll[1:1 + nsd] / Uhll[0]
+    v[d] = Ss
+    if Ss >= 0:
+        rK, uK, pK, EK, SK, qK, FK, side = rL, uL, pL, EL, SL, qL, FL, "star-left"
+    else:
+        rK, uK, pK, EK, SK, qK, FK, side = rR, uR, pR, ER, SR, qR, FR, "star-right"
+    fac = rK * (SK - uK[d]) / (SK - Ss)
+    Us = np.empty(nsd + 2)
+    Us[0] = fac
+    Us[1:1 + nsd] = fac * v
+    Us[1 + nsd] = fac * (EK / rK + (Ss - uK[d]) * (Ss + pK / (rK * (SK - uK[d]))))
+    return FK + SK * (Us - qK), max(abs(SL), abs(SR)), side
+
+
+@pytest.mark.parametrize("nsd", [1, 2, 3])
+def test_hll3_equals_toro_hllc(oracle, nsd):
+    """HLL3 is the HLLC solver: on random subsonic and supersonic pairs, every
+    branch (star-left, star-right, left, right) agrees with Toro's form of the
+    star states to rounding, which fixes the star pressure p* = p_K +
+    rho_K (u_K - S*)(u_K - S_K) inside E* (HLL3.hpp:88-103) and S*."""
+    rng = np.random.default_rng(11 + nsd)
+    seen = set()
+    for k in range(400):
+        mach = 0.6 if k % 4 else 2.5
+        qL, qR = random_pair(rng, nsd, mach)
+        for d in range(nsd):
+            F, ws = oracle.interface_flux(EQ_OF[nsd], "hll3", d, qL, qR)
+            Fe, wse, side = hllc_toro(qL, qR, nsd, d)
+            seen.add(side)
+            scale = max(1.0, np.abs(Fe).max())
+            np.testing.assert_allclose(F, Fe, rtol=0, atol=1e-12 * scale)
+            assert ws == pytest.approx(wse, rel=1e-14)
+    assert seen == {"left", "right", "star-left", "star-right"}
+
+
+@pytest.mark.parametrize("flux", ["hll", "hll3"])
+@pytest.mark.parametrize("nsd", [1, 2, 3])
+def test_flux_reflection_symmetry(oracle, flux, nsd):
+    """Mirror symmetry of a Riemann solver: reflecting x_d -> -x_d swaps the
+    sides and negates the normal velocity, so F(L, R) = -M F(M R, M L) with M
+    negating the normal momentum, and the wave speed is unchanged.  For HLL3
+    an interface with S* > 0 takes the left star formula and its mirror the
+    right one, so the two star branches check each other."""
+    rng = np.random.default_rng(5 + nsd)
+    branches = {True: 0, False: 0}
+    for k in range(300):
+        qL, qR = random_pair(rng, nsd, 0.6 if k % 3 else 2.0)
+        for d in range(nsd):
+            M = np.ones(nsd + 2)
+            M[1 + d] = -1.0
+            F, ws = oracle.interface_flux(EQ_OF[nsd], flux, d, qL, qR)
+            Fm, wsm = oracle.interface_flux(EQ_OF[nsd], flux, d, M * qR, M * qL)
+            scale = max(1.0, np.abs(F).max())
+            np.testing.assert_allclose(F, -M * Fm, rtol=0, atol=1e-12 * scale)
+            assert ws == pytest.approx(wsm, rel=1e-14)
+            if flux == "hll3":
+                _, _, side = hllc_toro(qL, qR, nsd, d)
+                if side.startswith("star"):
+                    branches[side == "star-left"] += 1
+    if flux == "hll3":
+        assert branches[True] > 50 and branches[False] > 50

@@ -122,6 +122,74 @@ def test_eno_exact_on_polynomials(oracle):
         np.testing.assert_allclose(right[0, 0, 0, inner], f(b[3:n - 3]), rtol=0, atol=1e-13)
 
 
+def _eno2_ghosted(nx, ny):
+    """euler3 volume of ones (E = 10), ENO2 ghost width 2, as EnoTest fills it."""
+    g = 2
+    shape = (1, ny + 2 * g if ny > 1 else 1, nx + 2 * g)
+    U = np.ones((5,) + shape)
+    U[4] = 10.0
+    return U
+
+
+@pytest.mark.parametrize("case", ["simple", "reverse", "reverse_y"])
+def test_eno2_shift_known_answers(oracle, case):
+    """EnoTest.ReconstructionSimple, ReconstructionSimpleReverse and
+    ReconstructionSimpleReverseYDirection (test/library_tests/src/fvm/
+    reconstruction/EnoTest.cpp:218-327): three rho values around ghosted
+    index 2 fix the direction of the stencil shift (ENOCPU.cpp:139-150) and
+    the coefficient rows used for each face, exactly."""
+    if case == "reverse_y":
+        nx, ny = 10, 10
+        U = _eno2_ghosted(nx, ny)
+        U[0, 0, 1, 3], U[0, 0, 2, 3], U[0, 0, 3, 3] = 1.0, 0.0, 2.0
+        cfg = oracle.make_config("euler3", "eno2", "hll3", "forwardeuler", "periodic", (nx, ny, 1))
+        left, right = oracle.reconstruct(cfg, U, 1)
+        assert right[0, 0, 2, 3] == -1.0 / 2.0
+        assert left[0, 0, 2, 3] == 1.0 / 2.0
+        return
+    nx = 10
+    U = _eno2_ghosted(nx, 1)
+    vals = (2.0, 0.0, 1.0) if case == "simple" else (1.0, 0.0, 2.0)
+    U[0, 0, 0, 1:4] = vals
+    cfg = oracle.make_config("euler3", "eno2", "hll3", "forwardeuler", "periodic", (nx, 1, 1))
+    left, right = oracle.reconstruct(cfg, U, 0)
+    if case == "simple":
+        assert right[0, 0, 0, 2] == 1.0 / 2.0
+        assert left[0, 0, 0, 2] == -1.0 / 2.0
+    else:
+        assert right[0, 0, 0, 2] == -1.0 / 2.0
+        assert left[0, 0, 0, 2] == 1.0 / 2.0
+
+
+@pytest.mark.parametrize("vals,shift,left_val,right_val", [
+    # level 0: |1| < |2| -> shift 1; level 1: |1| < |-1| false -> shift 1
+    ((0.0, 0.0, 1.0, 3.0, 4.0), 1, 1.0 / 3.0, 11.0 / 6.0),
+    # level 0: |0.5| < |1.5| -> 1; level 1: |1| < |2.5| -> 2
+    ((0.0, 1.0, 1.5, 3.0, 7.0), 2, 4.0 / 3.0, 19.0 / 12.0),
+    # level 0: |2| < |1| false -> 0; level 1: |-1| < |-0.5| false -> 0
+    ((5.0, 1.0, 3.0, 4.0, 4.5), 0, 7.0 / 3.0, 43.0 / 12.0),
+])
+def test_eno3_shift_hand_computed(oracle, vals, shift, left_val, right_val):
+    """ENO3 by the reference's rule (ENOCPU.cpp:139-150): at every level the
+    undivided differences at (cell-1, cell) are compared and the shift grows
+    by one when the left one is smaller in magnitude; the faces use the
+    coefficient rows shift and shift+1 (ENOCoefficients.cpp:35-40) on cells
+    i-shift .. i-shift+2.  Expected values worked out by hand for the five
+    indicator values u[i-2..i+2]."""
+    nx = 10
+    g = 3
+    U = np.ones((5, 1, 1, nx + 2 * g))
+    U[4] = 10.0
+    i = 6
+    U[0, 0, 0, i - 2:i + 3] = vals
+    cfg = oracle.make_config("euler3", "eno3", "hll3", "forwardeuler", "periodic", (nx, 1, 1))
+    left, right = oracle.reconstruct(cfg, U, 0)
+    assert left[0, 0, 0, i] == pytest.approx(left_val, rel=1e-14, abs=1e-14)
+    assert right[0, 0, 0, i] == pytest.approx(right_val, rel=1e-14, abs=1e-14)
+    # momentum variables are constant 1: every stencil reproduces them
+    assert left[1, 0, 0, i] == pytest.approx(1.0, rel=1e-14)
+
+
 # --------------------------------------------------------------------------
 # equations and fluxes
 

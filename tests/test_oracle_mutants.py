@@ -1,8 +1,10 @@
 """The oracle's pins catch plausible mistakes.  Each mutant is a copy of
 oracle/oracle.cpp with one planted error (a dropped term, a wrong sign,
-index or coefficient), compiled on the CPU; the pin suites
-(test_oracle_known_answers, test_oracle_closed_forms, test_golden) must fail
-against every mutant."""
+index or coefficient), compiled on the CPU; the independent pin suites
+(test_oracle_known_answers: the reference's own known answers;
+test_oracle_closed_forms: exact solutions, textbook forms and symmetries)
+must fail against every mutant.  The golden trajectories are left out: the
+oracle wrote them, so they pin nothing about it."""
 import concurrent.futures
 import os
 import shutil
@@ -19,6 +21,10 @@ MUTANTS = {
                            "const real a0R = 1 / (3 * (eps + b0) * (eps + b0));"),
     "hll3_star_pressure_sign": ("const real pm = pl + L.rho * (L.u[DIR] - sm) * udl;",
                                 "const real pm = pl - L.rho * (L.u[DIR] - sm) * udl;"),
+    "hll3_star_pressure_sign_right": ("const real pm = pr + R.rho * (R.u[DIR] - sm) * udr;",
+                                      "const real pm = pr - R.rho * (R.u[DIR] - sm) * udr;"),
+    "hll3_contact_pressure_sign": ("const real sm = (R.m[DIR] * udr - L.m[DIR] * udl + pr - pl) / aa;",
+                                   "const real sm = (R.m[DIR] * udr - L.m[DIR] * udl + pl - pr) / aa;"),
     "hll3_us_swapped_speeds": ("us[c] = (fr[1 + c] - fl[1 + c] - R.m[c] * sr + L.m[c] * sl) / aa;",
                                "us[c] = (fr[1 + c] - fl[1 + c] - R.m[c] * sl + L.m[c] * sr) / aa;"),
     "hll_jump_sign": ("F[i] = (sr * fl[i] - sl * fr[i] + srsl * (ur[i] - ul[i])) / den;",
@@ -77,6 +83,6 @@ def test_unmutated_source_matches_patterns():
 def test_pins_catch_mutant(mutant_libs, name):
     env = dict(os.environ, ORACLE_LIB=mutant_libs[name], OMP_NUM_THREADS="1")
     r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider",
-                        "tests/test_oracle_known_answers.py", "tests/test_oracle_closed_forms.py",
-                        "tests/test_golden.py"], cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+                        "tests/test_oracle_known_answers.py", "tests/test_oracle_closed_forms.py"],
+                       cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode != 0, "mutant %s passed every pin:\n%s" % (name, r.stdout[-2000:])
