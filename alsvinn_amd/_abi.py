@@ -40,6 +40,12 @@ SYMBOLS = [
     "avh_stream",
     "avh_compute_flux",
     "avh_reconstruct",
+    "avh_numerical_flux",
+    "avh_reconstruction",
+    "avh_apply_boundary",
+    "avh_obeys_constraints",
+    "avh_substep",
+    "avh_arith_selftest",
     "avh_batch_create",
     "avh_batch_destroy",
     "avh_batch_add_fixed_interval_adjuster",
@@ -118,6 +124,15 @@ class AvhProgress(ctypes.Structure):
     ]
 
 
+class AvhVolume(ctypes.Structure):
+    _fields_ = [
+        ("var", ctypes.c_void_p * 5),
+        ("nvars", ctypes.c_int),
+        ("n", ctypes.c_int * 3),
+        ("ghosts", ctypes.c_int * 3),
+    ]
+
+
 class AvhGroup(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int * 3), ("nsamples", ctypes.c_int)]
 
@@ -180,6 +195,14 @@ def load():
         "avh_synchronize": [vp],
         "avh_compute_flux": [vp, vp, P(d)],
         "avh_reconstruct": [vp, ctypes.c_int, vp, vp],
+        "avh_arith_selftest": [vp, ctypes.c_int, ctypes.c_uint64, i64, P(i64), P(d)],
+        "avh_numerical_flux": [vp, P(AvhConfig), P(AvhVolume), P(AvhVolume), P(ctypes.c_int), P(ctypes.c_int),
+                               P(d), vp],
+        "avh_reconstruction": [vp, P(AvhConfig), P(AvhVolume), ctypes.c_int, ctypes.c_int, P(AvhVolume),
+                               P(AvhVolume), P(ctypes.c_int), P(ctypes.c_int), vp],
+        "avh_apply_boundary": [vp, ctypes.c_int, P(AvhVolume), vp],
+        "avh_obeys_constraints": [vp, P(AvhConfig), P(AvhVolume), P(ctypes.c_int), vp],
+        "avh_substep": [vp, P(AvhConfig), P(P(AvhVolume)), ctypes.c_int, P(AvhVolume), d, d, P(d), vp],
         "avh_time_step_kernels": [vp, i64, P(d), P(ctypes.c_int)],
         "avh_batch_create": [vp, P(AvhConfig), ctypes.c_int, P(AvhGroup), d, ctypes.c_int, ctypes.c_int, P(vp)],
         "avh_batch_destroy": [vp],

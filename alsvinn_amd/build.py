@@ -20,6 +20,7 @@ SOURCES = [
     "avh_capi.hip",
     "avh_uq.hip",
     "avh_mlp.hip",
+    "avh_volume.hip",
     "avh_stage_burgers.hip",
     "avh_stage_euler1.hip",
     "avh_stage_euler2.hip",

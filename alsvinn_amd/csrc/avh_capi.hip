@@ -112,6 +112,77 @@ __global__ __launch_bounds__(256) void ghost_fill_kernel(const double* in, doubl
     }
 }
 
+// Arithmetic self-test: the stage kernels' square root without range fix-ups
+// (sqrt_core) and shared-reciprocal quotients (div_recip / div_shared)
+// against the compiler's sqrt and divide, bit for bit, on operands drawn
+// from a counter-based generator over the ranges the kernels' guards admit.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// a double in [2^elo, 2^ehi]: one draw in 64 is exactly one of the two
+// bounds, the others have a uniformly random 52-bit mantissa and a binary
+// exponent drawn uniformly from [elo, ehi - 1]
+__device__ __forceinline__ double draw_double(unsigned long long r, int elo, int ehi) {
+    if (((r >> 46) & 63ull) == 0) {
+        const int e = ((r >> 45) & 1ull) ? ehi : elo;
+        return __longlong_as_double((long long)((unsigned long long)(e + 1023) << 52));
+    }
+    const unsigned long long mant = r & ((1ull << 52) - 1);
+    const int e = elo + (int)((r >> 52) % (unsigned long long)(ehi - elo));
+    return __longlong_as_double((long long)(((unsigned long long)(e + 1023) << 52) | mant));
+}
+
+enum { ARITH_SQRT = 0, ARITH_DIV_RHO = 1, ARITH_DIV_WENO = 2, ARITH_DIV_ROE = 3 };
+
+__global__ __launch_bounds__(256) void arith_selftest_kernel(int kind, unsigned long long seed, long long n,
+                                                             unsigned long long* bad, double* first) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const unsigned long long r0 = splitmix64(seed ^ (2ull * (unsigned long long)i));
+        const unsigned long long r1 = splitmix64(seed ^ (2ull * (unsigned long long)i + 1ull));
+        double a = 0, b = 0, got = 0, want = 0;
+        if (kind == ARITH_SQRT) {
+            // [2^-767, 2^1023] and DBL_MAX: the range SharedOps::sqrt_ admits
+            a = ((r1 & 255ull) == 0) ? 0x1.fffffffffffffp+1023 : draw_double(r0, -767, 1023);
+            got = sqrt_core(a);
+            want = sqrt(a);
+        } else if (kind == ARITH_DIV_RHO) {
+            // denominators in [2^-100, 2^100], numerators zero or +-[2^-800, 2^600]:
+            // the ranges of make_all_ops / div_rho
+            b = draw_double(r1, -100, 100);
+            a = (r0 & 1023ull) == 0 ? 0.0 : draw_double(r0, -800, 600);
+            if (r1 & (1ull << 63)) a = -a;
+            got = div_shared(a, b, div_recip(b));
+            want = a / b;
+        } else if (kind == ARITH_DIV_WENO) {
+            // weights a0, a1 in [2^-400, 2^53]: quotient a0 / (a0 + a1)
+            const double a0 = draw_double(r0, -400, 53);
+            const double a1 = draw_double(r1, -400, 53);
+            a = a0;
+            b = a0 + a1;
+            got = div_shared(a, b, div_recip(b));
+            want = a0 / (a0 + a1);
+        } else {
+            // Roe denominators sqrt(rhoL) + sqrt(rhoR) in [2^-50, 2^51],
+            // numerators zero or +-[2^-800, 2^600]
+            b = draw_double(r1, -50, 51);
+            a = (r0 & 1023ull) == 0 ? 0.0 : draw_double(r0, -800, 600);
+            if (r1 & (1ull << 62)) a = -a;
+            got = div_shared(a, b, div_recip(b));
+            want = a / b;
+        }
+        if (__double_as_longlong(got) != __double_as_longlong(want)) {
+            if (atomicAdd(bad, 1ull) == 0ull) {
+                first[0] = a;
+                first[1] = b;
+            }
+        }
+    }
+}
+
 typedef void (*EwLauncher)(const double*, const double*, double*, Control*, long long, int, double,
                            hipStream_t);
 
@@ -417,6 +488,10 @@ avh_status avh_ctx_create(int device, avh_ctx** ctx) {
 }
 
 avh_status avh_ctx_destroy(avh_ctx* ctx) {
+    if (ctx && ctx->scratch) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipFree(ctx->scratch);
+    }
     delete ctx;
     return AVH_OK;
 }
@@ -880,6 +955,36 @@ avh_status avh_stats_read(avh_stats* t, int save, int primitive, double* mean, d
     const long long c = t->count[save];
     for (size_t i = 0; i < n; ++i) variance[i] = c > 0 ? variance[i] / (double)c : 0.0;
     *count = c;
+    return AVH_OK;
+}
+
+avh_status avh_arith_selftest(avh_ctx* ctx, int kind, uint64_t seed, int64_t n, int64_t* mismatches,
+                              double* first_bad) {
+    if (!ctx || !mismatches) return fail(AVH_EINVAL, "null argument");
+    if (kind < ARITH_SQRT || kind > ARITH_DIV_ROE) return fail(AVH_EINVAL, "unknown self-test kind");
+    if (n < 0) return fail(AVH_EINVAL, "negative operand count");
+    AVH_HIP(hipSetDevice(ctx->device));
+    void* mem = nullptr;
+    if (hipMalloc(&mem, 64) != hipSuccess) return fail(AVH_ENOMEM, "device allocation failed");
+    unsigned long long* bad = (unsigned long long*)mem;
+    double* first = (double*)((char*)mem + 16);
+    hipError_t e = hipMemset(mem, 0, 64);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(arith_selftest_kernel, dim3(4096), dim3(256), 0, 0, kind, (unsigned long long)seed,
+                           (long long)n, bad, first);
+        e = hipGetLastError();
+    }
+    unsigned long long hbad = 0;
+    double hfirst[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpy(&hbad, bad, sizeof hbad, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hfirst, first, sizeof hfirst, hipMemcpyDeviceToHost);
+    (void)hipFree(mem);
+    if (e != hipSuccess) return fail(AVH_EHIP, std::string("arith self-test: ") + hipGetErrorString(e));
+    *mismatches = (int64_t)hbad;
+    if (first_bad) {
+        first_bad[0] = hfirst[0];
+        first_bad[1] = hfirst[1];
+    }
     return AVH_OK;
 }
 

@@ -14,6 +14,10 @@
 
 struct avh_ctx {
     int device;
+    // device scratch of the per-call Volume entry points (wave-speed maxima,
+    // constraint flag), allocated on first use; `lock` serialises those calls
+    std::mutex lock;
+    void* scratch = nullptr;
 };
 
 namespace avh {

@@ -81,6 +81,25 @@ struct StageParams {
     double gamma;
 };
 
+// The reference's Volume (alsfvm/src/volume/Volume.cpp:44-60): one array per
+// conserved variable including its ghost cells, x fastest.  A stage kernel
+// instantiated for it covers a box of the interior (P.n = the box's extent)
+// and reads the halo from the ghost cells exactly as the caller filled them
+// (boundary conditions or a neighbour's cells), as the reference's
+// NumericalFlux::computeFlux does.
+struct VolumeParams {
+    const double* in[5];
+    double* out[5];
+    int tot[3];    // cells per axis including ghosts
+    int ghost[3];  // ghost cells per axis of the volume
+    int org[3];    // first cell of the box, interior coordinates
+};
+
+__device__ __forceinline__ long long volume_index(const VolumeParams& V, int x, int y, int z) {
+    return ((long long)(z + V.org[2] + V.ghost[2]) * V.tot[1] + (y + V.org[1] + V.ghost[1])) * V.tot[0] +
+           (x + V.org[0] + V.ghost[0]);
+}
+
 // Boundary map of a coordinate in [-G, n+G) onto the interior: periodic wrap
 // (alsfvm/include/alsfvm/boundary/Periodic.hpp:24-42) or neumann mirror
 // (alsfvm/include/alsfvm/boundary/Neumann.hpp:24-35).  Applied per axis this
@@ -278,13 +297,20 @@ struct StageKernel {
     }
 
     __device__ static void run(const StageParams& P) { run_tile(P, tile_id_xcd(blockIdx.x, gridDim.x)); }
+    __device__ static void run_volume(const StageParams& P, const VolumeParams& V) {
+        run_tile<true>(P, tile_id_xcd(blockIdx.x, gridDim.x), &V);
+    }
 
     // number of tiles covering a grid of n cells
     __host__ __device__ static unsigned int tiles(const int* n) {
         return (unsigned int)(((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ));
     }
 
-    __device__ static void run_tile(const StageParams& P, const unsigned int tile) {
+    // VOL: the cells live in a reference Volume (V) and P.n is the extent of
+    // the box the launch covers; otherwise in the interior-only buffers of P
+    // with the boundary map for the halo.
+    template <bool VOL = false>
+    __device__ static void run_tile(const StageParams& P, const unsigned int tile, const VolumeParams* V = nullptr) {
         __shared__ double lds[NV * LT + NV * NI_MAX];
         __shared__ double s_ws[3][NT / 64];
         double* su = lds;
@@ -320,16 +346,23 @@ struct StageKernel {
             gx = gx < n0 + HX ? gx : n0 + HX - 1;
             gy = gy < n1 + HY ? gy : n1 + HY - 1;
             gz = gz < n2 + HZ ? gz : n2 + HZ - 1;
-            gx = bc_map(gx, n0, P.bc);
-            gy = DIM >= 2 ? bc_map(gy, n1, P.bc) : 0;
-            gz = DIM >= 3 ? bc_map(gz, n2, P.bc) : 0;
-            sg[k] = ((long long)gz * n1 + gy) * n0 + gx;
+            if constexpr (VOL) {
+                sg[k] = volume_index(*V, gx, DIM >= 2 ? gy : 0, DIM >= 3 ? gz : 0);
+            } else {
+                gx = bc_map(gx, n0, P.bc);
+                gy = DIM >= 2 ? bc_map(gy, n1, P.bc) : 0;
+                gz = DIM >= 3 ? bc_map(gz, n2, P.bc) : 0;
+                sg[k] = ((long long)gz * n1 + gy) * n0 + gx;
+            }
         }
         double sv[SPT][NV];
 #pragma unroll
         for (int k = 0; k < SPT; ++k)
 #pragma unroll
-            for (int v = 0; v < NV; ++v) sv[k][v] = P.in[v * nc + sg[k]];
+            for (int v = 0; v < NV; ++v) {
+                if constexpr (VOL) sv[k][v] = V->in[v][sg[k]];
+                else sv[k][v] = P.in[v * nc + sg[k]];
+            }
         if (otf) {
 #pragma unroll
             for (int k = 0; k < SPT; ++k)
@@ -385,7 +418,8 @@ struct StageKernel {
             const int cz = c / (TX * TY);
             const int gx = ox + cx, gy = oy + cy, gz = oz + cz;
             inside[m] = (TC % NT == 0 || c < TC) && gx < n0 && gy < n1 && gz < n2;
-            gc[m] = inside[m] ? ((long long)gz * n1 + gy) * n0 + gx : 0;
+            if constexpr (VOL) gc[m] = inside[m] ? volume_index(*V, gx, gy, gz) : 0;
+            else gc[m] = inside[m] ? ((long long)gz * n1 + gy) * n0 + gx : 0;
         }
         const double ca = P.mode == MODE_RK3_2 ? 1. / 3. : (P.mode == MODE_RK3_3 ? 2.0 : 1.0);
         const double cb = P.mode == MODE_RK2_2 ? 0.5 : (P.mode == MODE_RK3_2 ? 3. / 4. : 1. / 3.);
@@ -425,7 +459,10 @@ struct StageKernel {
             }
             if (final_stage) bad = bad || !EQ::obeys(r, P.gamma);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) P.out[v * nc + gi] = r[v];
+            for (int v = 0; v < NV; ++v) {
+                if constexpr (VOL) V->out[v][gi] = r[v];
+                else P.out[v * nc + gi] = r[v];
+            }
         }
 
         if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
@@ -454,6 +491,12 @@ struct StageKernel {
 template <class K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void stage_kernel(StageParams P) {
     K::run(P);
+}
+
+// A stage over a box of a reference Volume (flux-only mode)
+template <class K>
+__global__ __launch_bounds__(256) void volume_stage_kernel(StageParams P, VolumeParams V) {
+    K::run_volume(P, V);
 }
 
 }  // namespace avh

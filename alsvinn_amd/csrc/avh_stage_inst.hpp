@@ -11,6 +11,8 @@ namespace avh {
 // `shape` selects one of the tile shapes below.
 typedef hipError_t (*StageLauncher)(const StageParams& P, int shape, hipStream_t stream);
 typedef hipError_t (*BatchLauncher)(const BatchParams& B, unsigned int nblocks, int shape, hipStream_t stream);
+// Flux-only stage over a box of a reference Volume (P.n = box extent)
+typedef hipError_t (*VolumeLauncher)(const StageParams& P, const VolumeParams& V, hipStream_t stream);
 
 // Tile shapes the stage kernels are built for; the autotuner times them on
 // the solver's own grid and keeps the fastest.  Every shape computes each
@@ -123,9 +125,36 @@ hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
     }
 }
 
+// The reference-Volume kernels: one tile per dimension -- 1D 256 cells, 2D
+// 32x15, 3D 8x4x4 bricks on 128 threads (the ghost-width-3 halo of a brick
+// still fits one workgroup's LDS for euler3) -- since this path serves the
+// reference's unfused per-call interfaces.
+template <class EQ, int RECON, class FLUX, int DIM>
+struct VolumeKernelFor;
+template <class EQ, int RECON, class FLUX>
+struct VolumeKernelFor<EQ, RECON, FLUX, 1> {
+    typedef StageKernel<EQ, RECON, FLUX, 1, 256, 1, 1, 256> type;
+};
+template <class EQ, int RECON, class FLUX>
+struct VolumeKernelFor<EQ, RECON, FLUX, 2> {
+    typedef StageKernel<EQ, RECON, FLUX, 2, 32, 15, 1, 256> type;
+};
+template <class EQ, int RECON, class FLUX>
+struct VolumeKernelFor<EQ, RECON, FLUX, 3> {
+    typedef StageKernel<EQ, RECON, typename FLUX::Plain, 3, 8, 4, 4, 128> type;
+};
+
+template <class EQ, int RECON, class FLUX, int DIM>
+hipError_t launch_volume_stage(const StageParams& P, const VolumeParams& V, hipStream_t stream) {
+    typedef typename VolumeKernelFor<EQ, RECON, FLUX, DIM>::type K;
+    hipLaunchKernelGGL(volume_stage_kernel<K>, dim3(K::tiles(P.n)), dim3(K::NTHREADS), 0, stream, P, V);
+    return hipGetLastError();
+}
+
 struct Launchers {
     StageLauncher stage = nullptr;
     BatchLauncher batch = nullptr;
+    VolumeLauncher volume = nullptr;
 };
 
 template <class EQ, class FLUX, int DIM>
@@ -135,18 +164,22 @@ Launchers pick_recon(int recon) {
     case RECON_NONE:
         L.stage = &launch_stage<EQ, RECON_NONE, FLUX, DIM>;
         L.batch = &launch_batch_stage<EQ, RECON_NONE, FLUX, DIM>;
+        L.volume = &launch_volume_stage<EQ, RECON_NONE, FLUX, DIM>;
         break;
     case RECON_WENO2:
         L.stage = &launch_stage<EQ, RECON_WENO2, FLUX, DIM>;
         L.batch = &launch_batch_stage<EQ, RECON_WENO2, FLUX, DIM>;
+        L.volume = &launch_volume_stage<EQ, RECON_WENO2, FLUX, DIM>;
         break;
     case RECON_ENO2:
         L.stage = &launch_stage<EQ, RECON_ENO2, FLUX, DIM>;
         L.batch = &launch_batch_stage<EQ, RECON_ENO2, FLUX, DIM>;
+        L.volume = &launch_volume_stage<EQ, RECON_ENO2, FLUX, DIM>;
         break;
     case RECON_ENO3:
         L.stage = &launch_stage<EQ, RECON_ENO3, FLUX, DIM>;
         L.batch = &launch_batch_stage<EQ, RECON_ENO3, FLUX, DIM>;
+        L.volume = &launch_volume_stage<EQ, RECON_ENO3, FLUX, DIM>;
         break;
     }
     if (DIM == 3) L.batch = nullptr;

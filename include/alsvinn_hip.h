@@ -198,6 +198,94 @@ avh_status avh_compute_flux(avh_solver* solver, double* host_interior_out, doubl
  * kernels fuse this step).  Synchronises. */
 avh_status avh_reconstruct(avh_solver* solver, int direction, double* host_left, double* host_right);
 
+/* ---------------------------------------------------------------------------
+ * The reference's per-call plug-in interfaces on device Volumes.
+ *
+ * avh_volume describes a reference Volume in device memory: one array per
+ * conserved variable (Volume::getScalarMemoryArea(v)->getPointer()), each
+ * (n[0]+2*ghosts[0]) * (n[1]+2*ghosts[1]) * (n[2]+2*ghosts[2]) doubles, x
+ * fastest, ghosts[d] = 0 on inactive axes (alsfvm/src/volume/Volume.cpp:44-60).
+ * Ghost cells are read as the caller filled them.  These are the calls the
+ * hip classes behind the reference's factory dispatch make (HipNumericalFlux,
+ * HipReconstruction, HipBoundary, HipCellComputer, HipIntegrator).  Each
+ * runs on `stream` (a hipStream_t; NULL: the null stream) and returns after
+ * the work is done.  Calls that share a context serialise on it.
+ * ------------------------------------------------------------------------- */
+typedef struct avh_volume {
+    double* var[5]; /* device array of each conserved variable (nvars used) */
+    int nvars;
+    int n[3];       /* interior cells per axis, 1 on inactive axes */
+    int ghosts[3];  /* ghost cells per axis, 0 on inactive axes */
+} avh_volume;
+
+/* NumericalFlux::computeFlux(in, waveSpeed, computeWaveSpeed, out, start,
+ * end) (alsfvm/include/alsfvm/numflux/NumericalFlux.hpp:44-52,
+ * alsfvm/src/numflux/NumericalFluxCPU.cpp:170-221): the cells of `out` whose
+ * interior coordinates lie in [start[d], n[d] + end[d]) receive the net flux
+ * sum over active d of ((-F(i+1/2)) + F(i-1/2)) (not divided by the cell
+ * size), and ws[d] the largest wave speed over the interfaces of that box
+ * along d (host array).  start >= 0 and end <= 0 on active axes, 0 on
+ * inactive ones; the volumes need at least the reconstruction's ghost
+ * cells.  cfg supplies equation, reconstruction, flux and gamma. */
+avh_status avh_numerical_flux(avh_ctx* ctx, const avh_config* cfg, const avh_volume* in, avh_volume* out,
+                              const int start[3], const int end[3], double ws[3], void* stream);
+
+/* Reconstruction::performReconstruction(in, direction, indicatorVariable,
+ * left, right, start, end) (alsfvm/include/alsfvm/reconstruction/
+ * Reconstruction.hpp:46-58, alsfvm/src/reconstruction/ReconstructionCPU.cpp:
+ * 37-97, ENOCPU.cpp:41-187): left/right receive the values at the lower /
+ * upper face along `direction` of every cell with interior coordinates in
+ * [start[d] - (d == direction), n[d] + end[d] + (d == direction)).  ENO
+ * chooses its stencils on `indicator_variable`; WENO2 weights on the
+ * equation's weight variable (rho / u), as the reference. */
+avh_status avh_reconstruction(avh_ctx* ctx, const avh_config* cfg, const avh_volume* in, int direction,
+                              int indicator_variable, avh_volume* left, avh_volume* right,
+                              const int start[3], const int end[3], void* stream);
+
+/* Boundary::applyBoundaryConditions(volume, grid) (alsfvm/include/alsfvm/
+ * boundary/Boundary.hpp:30-31, alsfvm/src/boundary/BoundaryCPU.cpp:30-121):
+ * every ghost cell of every variable, corners included, from the interior
+ * by the periodic or neumann map (AVH_BC_*), in place. */
+avh_status avh_apply_boundary(avh_ctx* ctx, int boundary, avh_volume* volume, void* stream);
+
+/* CellComputer::obeysConstraints(conserved, extra) (alsfvm/include/alsfvm/
+ * equation/CellComputer.hpp:61, alsfvm/src/equation/CPUCellComputer.cpp:
+ * 79-125): *obeys = 1 if every cell, ghost cells included, has finite
+ * conserved and derived variables and (euler) rho > 0 and p > 0. */
+avh_status avh_obeys_constraints(avh_ctx* ctx, const avh_config* cfg, const avh_volume* volume, int* obeys,
+                                 void* stream);
+
+/* Integrator::performSubstep(inputs, cellLengths, dt, cfl, out, substep,
+ * timestepInformation) -> dt (alsfvm/include/alsfvm/integrator/
+ * Integrator.hpp:76-80; ForwardEuler.cpp:32-51, RungeKutta2.cpp:37-67,
+ * RungeKutta3.cpp:49-86) for cfg->integrator: out = the system applied to
+ * inputs[substep] over the whole interior (ConservedSystem::operator(),
+ * alsfvm/src/simulator/ConservedSystem.cpp:31-103), then the substep's
+ * combination with inputs[0..substep] on every cell of `out`, ghosts
+ * included, as the reference's Volume operators do.  At substep 0 the step
+ * is dt = cfl / sum_d(ws_d / dx_d) (Integrator::computeTimestep,
+ * alsfvm/src/integrator/Integrator.cpp:20-46) with dx from cfg's corners,
+ * then min(dt, dt_cap) -- dt_cap carries the caller's timestep adjusters
+ * (FixedIntervalWriter: next save time - t; +inf if none); later substeps
+ * use dt as given.  *dt_out receives the dt used. */
+avh_status avh_substep(avh_ctx* ctx, const avh_config* cfg, const avh_volume* const* inputs, int substep,
+                       avh_volume* out, double dt, double dt_cap, double* dt_out, void* stream);
+
+/* Device self-test of the FP64 arithmetic the stage kernels use in place of
+ * the compiler's sqrt and divide where the operands admit it: n operands
+ * drawn from a counter-based generator (seed) over the admitted range, each
+ * result compared bit for bit with sqrt() or '/' on the device.
+ * kind 0: sqrt over [2^-767, DBL_MAX]; 1: shared-reciprocal quotient with
+ * denominators in [2^-100, 2^100] and numerators 0 or +-[2^-800, 2^600];
+ * 2: WENO2 weight a0 / (a0 + a1), a0, a1 in [2^-400, 2^53];
+ * 3: Roe quotient, denominators in [2^-50, 2^51].  Bounds are included.
+ * mismatches = count of
+ * differing results, first_bad[2] (may be NULL) = operands of one of them.
+ * No reference counterpart (the reference divides and takes square roots
+ * directly; this checks that the shortcuts give the same bits). */
+avh_status avh_arith_selftest(avh_ctx* ctx, int kind, uint64_t seed, int64_t n, int64_t* mismatches,
+                              double* first_bad);
+
 /* ------------------------------------------------------------------
  * Batched Monte Carlo (alsuq on the hip platform).
  *

@@ -968,6 +968,25 @@ int oracle_compute_flux(const avh_config* cfg, const double* U, double* out, dou
     }
 }
 
+/* NumericalFlux::computeFlux(U, ws, true, out, start, end) on a
+ * caller-provided ghosted state and output (NumericalFluxCPU.cpp:170-221). */
+int oracle_compute_flux_box(const avh_config* cfg, const double* U, double* out, double* ws, const int* start,
+                            const int* end) {
+    try {
+        std::unique_ptr<Simulator> s(make_sim(*cfg));
+        std::memcpy(s->vols[0].d.data(), U, s->vols[0].d.size() * sizeof(double));
+        std::memcpy(s->vols[1].d.data(), out, s->vols[1].d.size() * sizeof(double));
+        ws[0] = ws[1] = ws[2] = 0;
+        s->system.flux->compute(s->vols[0], ws, s->vols[1], ivec3{start[0], start[1], start[2]},
+                                ivec3{end[0], end[1], end[2]});
+        std::memcpy(out, s->vols[1].d.data(), s->vols[1].d.size() * sizeof(double));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return AVH_EINVAL;
+    }
+}
+
 /* ConservedSystem::operator() on a ghosted state (interior + strips). */
 int oracle_system(const avh_config* cfg, const double* U, double* out, double* ws, int whole_domain) {
     try {
@@ -1000,6 +1019,32 @@ int oracle_reconstruct(const avh_config* cfg, const double* U, int dir, double* 
         std::memcpy(L.d.data(), left, L.d.size() * sizeof(double));
         std::memcpy(R.d.data(), right, R.d.size() * sizeof(double));
         r->reconstruct(in, dir, 0, L, R, ivec3{}, ivec3{});
+        std::memcpy(left, L.d.data(), L.d.size() * sizeof(double));
+        std::memcpy(right, R.d.data(), R.d.size() * sizeof(double));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return AVH_EINVAL;
+    }
+}
+
+/* Reconstruction::performReconstruction(in, dir, indicator, left, right,
+ * start, end) on caller-provided ghosted volumes. */
+int oracle_reconstruct_box(const avh_config* cfg, const double* U, int dir, int indicator, double* left,
+                           double* right, const int* start, const int* end) {
+    try {
+        validate(*cfg);
+        const int dim = active_dim(*cfg);
+        auto r = make_recon(cfg->reconstruction, dim);
+        const int g = std::max(1, r->ghosts());
+        const int nv = num_vars(cfg->equation);
+        if (indicator < 0 || indicator >= nv) throw std::runtime_error("no such indicator variable");
+        Volume in(nv, cfg->n[0], cfg->n[1], cfg->n[2], g, dim > 1 ? g : 0, dim > 2 ? g : 0);
+        std::memcpy(in.d.data(), U, in.d.size() * sizeof(double));
+        Volume L = in, R = in;
+        std::memcpy(L.d.data(), left, L.d.size() * sizeof(double));
+        std::memcpy(R.d.data(), right, R.d.size() * sizeof(double));
+        r->reconstruct(in, dir, indicator, L, R, ivec3{start[0], start[1], start[2]}, ivec3{end[0], end[1], end[2]});
         std::memcpy(left, L.d.data(), L.d.size() * sizeof(double));
         std::memcpy(right, R.d.data(), R.d.size() * sizeof(double));
         return 0;

@@ -100,6 +100,10 @@ def lib():
         L.oracle_compute_flux.argtypes = [P(OracleConfig), P(d), P(d), P(d)]
         L.oracle_system.argtypes = [P(OracleConfig), P(d), P(d), P(d), ctypes.c_int]
         L.oracle_reconstruct.argtypes = [P(OracleConfig), P(d), ctypes.c_int, P(d), P(d)]
+        ivec = ctypes.c_int * 3
+        L.oracle_compute_flux_box.argtypes = [P(OracleConfig), P(d), P(d), P(d), ivec, ivec]
+        L.oracle_reconstruct_box.argtypes = [P(OracleConfig), P(d), ctypes.c_int, ctypes.c_int, P(d), P(d),
+                                             ivec, ivec]
         L.oracle_interface_flux.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, d, P(d), P(d), P(d)]
         L.oracle_interface_flux.restype = d
         L.oracle_point_flux.argtypes = [ctypes.c_int, ctypes.c_int, d, P(d), P(d)]
@@ -248,6 +252,31 @@ def compute_flux(cfg, U_ghosted, out_ghosted):
     if rc:
         raise OracleError(lib().oracle_last_error().decode())
     return out, ws
+
+
+def compute_flux_box(cfg, U_ghosted, out_ghosted, start, end):
+    """NumericalFlux::computeFlux with the reference's start/end offsets."""
+    U = np.ascontiguousarray(U_ghosted, dtype=np.float64)
+    out = np.ascontiguousarray(out_ghosted, dtype=np.float64).copy()
+    ws = np.zeros(3)
+    rc = lib().oracle_compute_flux_box(ctypes.byref(cfg), _dp(U), _dp(out), _dp(ws), (ctypes.c_int * 3)(*start),
+                                       (ctypes.c_int * 3)(*end))
+    if rc:
+        raise OracleError(lib().oracle_last_error().decode())
+    return out, ws
+
+
+def reconstruct_box(cfg, U_ghosted, direction, indicator, left, right, start, end):
+    """Reconstruction::performReconstruction with an indicator variable and
+    start/end offsets; left/right are the volumes written into."""
+    U = np.ascontiguousarray(U_ghosted, dtype=np.float64)
+    left = np.ascontiguousarray(left, dtype=np.float64).copy()
+    right = np.ascontiguousarray(right, dtype=np.float64).copy()
+    rc = lib().oracle_reconstruct_box(ctypes.byref(cfg), _dp(U), int(direction), int(indicator), _dp(left),
+                                      _dp(right), (ctypes.c_int * 3)(*start), (ctypes.c_int * 3)(*end))
+    if rc:
+        raise OracleError(lib().oracle_last_error().decode())
+    return left, right
 
 
 def system(cfg, U_ghosted, out_ghosted, whole_domain=False):
