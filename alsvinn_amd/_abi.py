@@ -34,6 +34,7 @@ SYMBOLS = [
     "avh_synchronize",
     "avh_set_graph_steps",
     "avh_prepare_graph",
+    "avh_step_noalloc",
     "avh_set_tile_shape",
     "avh_autotune",
     "avh_time_step_kernels",
@@ -182,6 +183,7 @@ def load():
         "avh_enqueue_steps": [vp, i64],
         "avh_set_graph_steps": [vp, ctypes.c_int],
         "avh_prepare_graph": [vp],
+        "avh_step_noalloc": [vp, P(AvhConfig), vp, d, vp, vp, vp],
         "avh_set_tile_shape": [vp, ctypes.c_int],
         "avh_autotune": [vp, P(ctypes.c_int), P(d), P(ctypes.c_int)],
         "avh_batch_set_tile_shape": [vp, ctypes.c_int],

@@ -64,10 +64,13 @@ def run(path, device=0, out=sys.stdout):
 
 
 def main(argv=None):
+    from .alsvinncli import add_decomposition_flags, check_decomposition
     ap = argparse.ArgumentParser(description="Run an alsuq configuration on the hip platform")
     ap.add_argument("input", help="XML configuration file")
     ap.add_argument("--device", type=int, default=0)
+    add_decomposition_flags(ap, sample=True)
     a = ap.parse_args(argv)
+    check_decomposition(ap, a, sample=True)
     run(a.input, a.device)
     return 0
 

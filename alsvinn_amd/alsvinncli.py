@@ -44,11 +44,37 @@ def run(path, device=0, out=sys.stdout):
     return sim, files
 
 
+def add_decomposition_flags(ap, sample=False):
+    """The reference's MPI decomposition options (alsvinncli/src/main.cpp:55-67,
+    alsuqcli/src/main.cpp:40-51), accepted so that existing command lines run
+    unchanged.  The hip platform runs on one GPU: each count must be 1."""
+    for axis in "xyz":
+        ap.add_argument("--multi-" + axis, type=int, default=1,
+                        help="processors in the %s direction (one GPU: must be 1)" % axis)
+    if not sample:
+        for axis in "xyz":
+            ap.add_argument("-" + axis, "--automatic-" + axis, action="store_true",
+                            help="divide all cores in the %s direction (one GPU: no effect)" % axis)
+    else:
+        ap.add_argument("--multi-sample", type=int, default=1,
+                        help="processors in the sample direction (one GPU: must be 1)")
+
+
+def check_decomposition(ap, a, sample=False):
+    names = ["multi_x", "multi_y", "multi_z"] + (["multi_sample"] if sample else [])
+    for name in names:
+        if getattr(a, name) != 1:
+            ap.error("--%s %d: the hip platform runs on one GPU, every decomposition count must be 1"
+                     % (name.replace("_", "-"), getattr(a, name)))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="Run an alsvinn configuration on the hip platform")
     ap.add_argument("input", help="XML configuration file")
     ap.add_argument("--device", type=int, default=0)
+    add_decomposition_flags(ap)
     a = ap.parse_args(argv)
+    check_decomposition(ap, a)
     run(a.input, a.device)
     return 0
 

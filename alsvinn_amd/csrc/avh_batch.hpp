@@ -1,8 +1,13 @@
 // Batched Monte Carlo: every sample of every resolution group advances in the
 // same launches.
 //
-// A launch covers all groups: blocks [first_block, first_block + nsamples *
-// tiles) belong to a group, and within it sample-major to (sample, tile).
+// A stage launch covers all groups with 8 * per_xcd blocks.  Workgroups are
+// dealt to the 8 XCDs round-robin (block b runs on XCD b % 8), and XCD x
+// takes every 8th sample -- samples x, x + 8, x + 16, ... of every group --
+// tile by tile: the tiles of a sample share one L2, and every XCD holds an
+// equal share of every resolution, so a group that finishes early leaves no
+// XCD idle.  Element-wise launches run blocks [first_block, first_block +
+// nsamples * tiles) per group, sample-major.
 // Each sample owns a Control block, so dt, time, save counters, the
 // constraint flag and the loop test are per sample, exactly as the
 // reference's Runner advances each sample with its own Simulator
@@ -42,7 +47,34 @@ struct BatchParams {
     int bc;
     int mode;
     double gamma;
+    unsigned int per_xcd;  // stage blocks per XCD
 };
+
+// samples s of a group of n with s % 8 == x
+__host__ __device__ __forceinline__ unsigned int xcd_samples(int n, unsigned int x) {
+    return (unsigned int)((n + 7 - (int)x) / 8);
+}
+
+// (group, sample, tile) of stage block b; false for the padding blocks of an
+// XCD whose share is below per_xcd
+__device__ __forceinline__ bool batch_stage_decode(const BatchParams& B, unsigned int b, int& gi,
+                                                   unsigned int& sample, unsigned int& tile) {
+    const unsigned int x = b % 8u;
+    unsigned int k = b / 8u;
+#pragma unroll
+    for (int i = 0; i < MAX_GROUPS; ++i) {
+        if (i >= B.ngroups) break;
+        const unsigned int cnt = xcd_samples(B.g[i].nsamples, x) * B.g[i].tiles;
+        if (k < cnt) {
+            gi = i;
+            sample = (k / B.g[i].tiles) * 8u + x;
+            tile = k % B.g[i].tiles;
+            return true;
+        }
+        k -= cnt;
+    }
+    return false;
+}
 
 __device__ __forceinline__ int batch_group(const BatchParams& B, unsigned int b) {
     int gi = 0;
@@ -54,12 +86,10 @@ __device__ __forceinline__ int batch_group(const BatchParams& B, unsigned int b)
 
 template <class K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void batch_stage_kernel(BatchParams B) {
-    const unsigned int b = tile_id_xcd(blockIdx.x, gridDim.x);
-    const int gi = batch_group(B, b);
+    int gi = 0;
+    unsigned int sample = 0, tile = 0;
+    if (!batch_stage_decode(B, blockIdx.x, gi, sample, tile)) return;
     const GroupDesc& G = B.g[gi];
-    const unsigned int local = b - G.first_block;
-    const unsigned int sample = local / G.tiles;
-    const unsigned int tile = local % G.tiles;
     StageParams P;
     P.in = G.in + (long long)sample * G.stride;
     P.u0 = G.u0 + (long long)sample * G.stride;

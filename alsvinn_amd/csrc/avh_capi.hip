@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -316,6 +317,79 @@ EwLauncher ew_for(int eq) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// The time step that allocates nothing (avh_step_noalloc).  Its two stage
+// buffers (L0 and U1/U2) and its control block live in `scratch`, a static
+// device array that every GPU the library runs on holds until the process
+// ends: 512 bytes for the control block, then two buffers of
+// NOALLOC_MAX_VALUES doubles (64 MiB in all).  Host code never hands the
+// array's name to a kernel; its device address is looked up once per device
+// with hipGetSymbolAddress when a context is created, and kept per device.
+constexpr long long NOALLOC_MAX_VALUES = 1ll << 22;  // nvars * cells of one stage buffer
+constexpr long long NOALLOC_CTRL_DOUBLES = 64;
+static_assert(sizeof(Control) <= NOALLOC_CTRL_DOUBLES * sizeof(double), "the control block fits its slot");
+static __device__ __align__(256) double scratch[NOALLOC_CTRL_DOUBLES + 2 * NOALLOC_MAX_VALUES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device (`device`)
+static avh_status noalloc_scratch(int device, double** out) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(AVH_EINVAL, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(AVH_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = (double*)p;
+    }
+    *out = g_scratch_ptr[device];
+    return AVH_OK;
+}
+
+// a fresh control block for one step: time 0, the CFL number and cell
+// lengths, and dt_cap as a one-save fixed-interval adjuster (next save at
+// 1 * dt_cap + 0, so the clip candidate is dt_cap exactly); every field
+// written, since the block keeps the previous call's contents
+static __global__ void noalloc_init_kernel(Control* c, double c0, double c1, double c2, double cfl, double cap) {
+    if (threadIdx.x != 0) return;
+    clear_wave_speeds(c);
+    c->t = 0.0;
+    c->dt = 0.0;
+    c->scale = 0.0;
+    c->last_dt = 0.0;
+    c->t_end = __builtin_inf();
+    c->cfl = cfl;
+    c->cell[0] = c0;
+    c->cell[1] = c1;
+    c->cell[2] = c2;
+    for (int i = 0; i < MAX_ADJ; ++i) {
+        c->adj_interval[i] = 0.0;
+        c->adj_start[i] = 0.0;
+        c->adj_saved[i] = 0;
+    }
+    c->n_adj = 0;
+    if (!__builtin_isinf(cap)) {
+        c->n_adj = 1;
+        c->adj_interval[0] = cap;
+        c->adj_saved[0] = 1;
+    }
+    c->steps = 0;
+    c->active = 1;
+    c->violated = 0;
+    c->saves_pending = 0;
+    c->pause_on_save = 0;
+    c->block_violation = 0;
+}
+
+static __global__ void noalloc_result_kernel(const Control* c, double* dt, int* violated) {
+    if (threadIdx.x != 0) return;
+    if (dt) dt[0] = c->last_dt;
+    if (violated) violated[0] = c->violated;
+}
+
 struct avh_solver {
     avh_ctx* ctx = nullptr;
     avh_config cfg{};
@@ -480,6 +554,10 @@ avh_status avh_ctx_create(int device, avh_ctx** ctx) {
     AVH_HIP(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) return fail(AVH_EINVAL, "no such device");
     AVH_HIP(hipSetDevice(device));
+    // the no-allocation step's scratch address on this device, once
+    double* sp = nullptr;
+    avh_status r = noalloc_scratch(device, &sp);
+    if (r) return r;
     auto c = new (std::nothrow) avh_ctx;
     if (!c) return fail(AVH_ENOMEM, "host allocation failed");
     c->device = device;
@@ -701,6 +779,96 @@ avh_status avh_enqueue_steps(avh_solver* s, int64_t steps) {
     avh_status r = use_device(s);
     if (r) return r;
     return enqueue_steps(s, steps);
+}
+
+avh_status avh_step_noalloc(avh_ctx* ctx, const avh_config* cfg, double* state, double dt_cap, double* dt_out,
+                            int* violated_out, void* stream) {
+    if (!ctx || !cfg || !state) return fail(AVH_EINVAL, "null argument");
+    const int nv = nvars_of(cfg->equation);
+    if (!nv) return fail(AVH_EINVAL, "unknown equation");
+    const int ng = ghosts_of(cfg->reconstruction);
+    if (!ng) return fail(AVH_EINVAL, "unknown reconstruction");
+    if (cfg->integrator < AVH_INT_FORWARDEULER || cfg->integrator > AVH_INT_RUNGEKUTTA3)
+        return fail(AVH_EINVAL, "unknown integrator");
+    if (cfg->boundary != AVH_BC_PERIODIC && cfg->boundary != AVH_BC_NEUMANN)
+        return fail(AVH_EINVAL, "unknown boundary");
+    for (int d = 0; d < 3; ++d)
+        if (cfg->n[d] < 1) return fail(AVH_EINVAL, "grid sizes must be >= 1");
+    const int dim = cfg->n[2] > 1 ? 3 : (cfg->n[1] > 1 ? 2 : 1);
+    for (int d = 0; d < dim; ++d)
+        if (cfg->n[d] < ng)
+            return fail(AVH_EINVAL, "too few cells: every active axis needs at least as many cells as ghost cells");
+    const long long ncells = (long long)cfg->n[0] * cfg->n[1] * cfg->n[2];
+    if ((long long)nv * ncells > NOALLOC_MAX_VALUES) {
+        char buf[200];
+        std::snprintf(buf, sizeof buf,
+                      "state of %lld values (nvars * cells) exceeds the no-allocation scratch of %lld values",
+                      (long long)nv * ncells, NOALLOC_MAX_VALUES);
+        return fail(AVH_EINVAL, buf);
+    }
+    const StageLauncher stage = launchers_for(cfg->equation, cfg->reconstruction, cfg->flux, dim).stage;
+    if (!stage)
+        return fail(AVH_EINVAL, "unsupported combination of equation, flux and grid dimension");
+    AVH_HIP(hipSetDevice(ctx->device));
+    double* sp = nullptr;
+    avh_status r = noalloc_scratch(ctx->device, &sp);
+    if (r) return r;
+    // tile shape: the autotuner's choice for this problem if a solver made
+    // one, else a fixed default (timing would need events)
+    int shape = dim == 1 ? 0 : (dim == 2 ? 3 : 1);
+    int tuned;
+    if (tune_lookup(TuneKey(0, cfg->equation * 16 + cfg->reconstruction, cfg->flux, dim, cfg->n[0], cfg->n[1],
+                            cfg->n[2]),
+                    &tuned))
+        shape = tuned;
+    Control* ctrl = (Control*)sp;
+    double* B = sp + NOALLOC_CTRL_DOUBLES;
+    double* C = B + NOALLOC_MAX_VALUES;
+    const hipStream_t st = (hipStream_t)stream;
+    double cell[3];
+    for (int d = 0; d < 3; ++d) cell[d] = (cfg->upper[d] - cfg->lower[d]) / double(cfg->n[d]);
+    if (cfg->n[1] == 1) cell[1] = 1;
+    if (cfg->n[2] == 1) cell[2] = 1;
+    hipLaunchKernelGGL(noalloc_init_kernel, dim3(1), dim3(64), 0, st, ctrl, cell[0], cell[1], cell[2], cfg->cfl,
+                       dt_cap);
+    AVH_HIP(hipGetLastError());
+    const EwLauncher ew = ew_for(cfg->equation);
+    StageParams P;
+    P.ctrl = ctrl;
+    P.ncells = ncells;
+    for (int d = 0; d < 3; ++d) P.n[d] = cfg->n[d];
+    P.bc = cfg->boundary;
+    P.gamma = cfg->gamma;
+    P.in = state;
+    P.u0 = state;
+    P.out = B;
+    P.mode = MODE_L;
+    AVH_HIP(stage(P, shape, st));
+    if (cfg->integrator == AVH_INT_FORWARDEULER) {
+        ew(B, state, state, ctrl, ncells, EW_FE_FINISH, cfg->gamma, st);
+        AVH_HIP(hipGetLastError());
+    } else if (cfg->integrator == AVH_INT_RUNGEKUTTA2) {
+        ew(B, state, C, ctrl, ncells, EW_RK2_U1, cfg->gamma, st);
+        AVH_HIP(hipGetLastError());
+        P.in = C;
+        P.out = state;
+        P.mode = MODE_RK2_2;
+        AVH_HIP(stage(P, shape, st));
+    } else {
+        P.in = B;
+        P.out = C;
+        P.mode = MODE_RK3_2;
+        AVH_HIP(stage(P, shape, st));
+        P.in = C;
+        P.out = state;
+        P.mode = MODE_RK3_3;
+        AVH_HIP(stage(P, shape, st));
+    }
+    hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, st, ctrl);
+    AVH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(noalloc_result_kernel, dim3(1), dim3(64), 0, st, ctrl, dt_out, violated_out);
+    AVH_HIP(hipGetLastError());
+    return AVH_OK;
 }
 
 avh_status avh_prepare_graph(avh_solver* s) {

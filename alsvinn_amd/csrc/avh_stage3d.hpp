@@ -90,7 +90,11 @@ __device__ __forceinline__ void recon_face_reg(const double (&u)[WZ][NV], const 
     }
 }
 
-template <class EQ, int RECON, class FLUX, int TX, int TY, int LZ, int NTH>
+// TWO_BARRIERS: the plane tile is double-buffered and the x- and y-interface
+// fluxes have LDS buffers of their own, so a plane takes two barriers (plane
+// published; fluxes published) instead of five, and the x, y and z flux
+// evaluations of a plane run in one barrier-free stretch.
+template <class EQ, int RECON, class FLUX, int TX, int TY, int LZ, int NTH, bool TWO_BARRIERS = false>
 struct MarchKernel3 {
     static constexpr int NV = EQ::NV;
     static constexpr int NC = TX * TY;  // columns: one per thread, threads past them work on the planes only
@@ -156,9 +160,13 @@ struct MarchKernel3 {
     }
 
     __device__ static void run(const StageParams& P) {
-        __shared__ double sp[NV * LP];
+        constexpr int NBUF = TWO_BARRIERS ? 2 : 1;
+        __shared__ double sp_buf[NBUF][NV * LP];
         __shared__ double sf[NV * NIMAX];
+        __shared__ double sf_y[TWO_BARRIERS ? NV * NIMAX : 1];
         __shared__ double s_ws[3][NT / 64];
+        double* const sfx = sf;
+        double* const sfy = TWO_BARRIERS ? sf_y : sf;
 
         Control* ctrl = P.ctrl;
         if (!ctrl->active && P.mode != MODE_FLUX_ONLY) return;
@@ -299,6 +307,7 @@ struct MarchKernel3 {
         }
 
         for (int k = z0; k < z1; ++k) {
+            double* const sp = sp_buf[TWO_BARRIERS ? ((k - z0) & 1) : 0];
             // plane k into LDS: own column from the window (w[HW-1] is plane k),
             // the halo ring from the prefetched registers
             if (has_col) {
@@ -324,33 +333,55 @@ struct MarchKernel3 {
             double acc[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-            plane_pass<0>(sp, sf, ox, oy, P.n, P.gamma, ws[0]);
-            __syncthreads();
-            if (has_col) {
-                const int ip = cy * (TX + 1) + cx + 1;
-#pragma unroll
-                for (int v = 0; v < NV; ++v) acc[v] = acc[v] + ((-sf[v * NIMAX + ip]) + sf[v * NIMAX + ip - 1]);
-            }
-            __syncthreads();
-            plane_pass<1>(sp, sf, ox, oy, P.n, P.gamma, ws[1]);
-            __syncthreads();
-            if (has_col) {
-                const int ip = (cy + 1) * TX + cx;
-#pragma unroll
-                for (int v = 0; v < NV; ++v) acc[v] = acc[v] + ((-sf[v * NIMAX + ip]) + sf[v * NIMAX + ip - TX]);
-            }
             // z: F(k+1/2) from R(k) (kept) and L(k+1) (window centred at k+1)
             double fz[NV], rnext[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) fz[v] = rnext[v] = 0.0;
-            if (has_col) {
-                double lz[NV];
-                recon_face_reg<RECON, NV, false, WZ>(w, HW, lz);
-                recon_face_reg<RECON, NV, true, WZ>(w, HW, rnext);
-                const typename EQ::All aL = EQ::make_all(rprev, P.gamma);
-                const typename EQ::All aR = EQ::make_all(lz, P.gamma);
-                const double wz = FLUX::template flux<2, EQ>(aL, aR, P.gamma, fz);
-                if (col_ok) ws[2] = ws_max(ws[2], wz);
+            auto z_flux = [&]() {
+                if (has_col) {
+                    double lz[NV];
+                    recon_face_reg<RECON, NV, false, WZ>(w, HW, lz);
+                    recon_face_reg<RECON, NV, true, WZ>(w, HW, rnext);
+                    const typename EQ::All aL = EQ::make_all(rprev, P.gamma);
+                    const typename EQ::All aR = EQ::make_all(lz, P.gamma);
+                    const double wz = FLUX::template flux<2, EQ>(aL, aR, P.gamma, fz);
+                    if (col_ok) ws[2] = ws_max(ws[2], wz);
+                }
+            };
+            auto add_x = [&]() {
+                if (has_col) {
+                    const int ip = cy * (TX + 1) + cx + 1;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v)
+                        acc[v] = acc[v] + ((-sfx[v * NIMAX + ip]) + sfx[v * NIMAX + ip - 1]);
+                }
+            };
+            auto add_y = [&]() {
+                if (has_col) {
+                    const int ip = (cy + 1) * TX + cx;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v)
+                        acc[v] = acc[v] + ((-sfy[v * NIMAX + ip]) + sfy[v * NIMAX + ip - TX]);
+                }
+            };
+            if constexpr (TWO_BARRIERS) {
+                // every flux of the plane, then one barrier, then the sums in
+                // the reference's x, y, z order
+                plane_pass<0>(sp, sfx, ox, oy, P.n, P.gamma, ws[0]);
+                plane_pass<1>(sp, sfy, ox, oy, P.n, P.gamma, ws[1]);
+                z_flux();
+                __syncthreads();
+                add_x();
+                add_y();
+            } else {
+                plane_pass<0>(sp, sfx, ox, oy, P.n, P.gamma, ws[0]);
+                __syncthreads();
+                add_x();
+                __syncthreads();
+                plane_pass<1>(sp, sfy, ox, oy, P.n, P.gamma, ws[1]);
+                __syncthreads();
+                add_y();
+                z_flux();
             }
 #pragma unroll
             for (int v = 0; v < NV; ++v) acc[v] = acc[v] + ((-fz[v]) + fprev[v]);
@@ -408,7 +439,9 @@ struct MarchKernel3 {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) w[WZ - 1][v] = otf ? cin[v] * s + cu0[v] : cin[v];
             }
-            __syncthreads();
+            // one plane buffer: every thread must be done reading it (and the
+            // flux buffer) before the next plane is written
+            if constexpr (!TWO_BARRIERS) __syncthreads();
         }
 
 #undef AVH_PREFETCH_RING

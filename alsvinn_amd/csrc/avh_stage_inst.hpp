@@ -29,13 +29,17 @@ typedef hipError_t (*VolumeLauncher)(const StageParams& P, const VolumeParams& V
 //       24x20 (500 / 504) and 16x15 (255 / 256), against 32x16 (528 / 544).
 //   3D: columns marching over 32 planes, 256 threads: 32x8 (264 / 288
 //       plane interfaces), 16x15 or 32x7 (at most 256 each: one turn; the
-//       threads past the columns only work on the planes).
-constexpr int NSHAPES1 = 2, NSHAPES2 = 5, NSHAPES3 = 3;
+//       threads past the columns only work on the planes); then 16x15 over
+//       32 and over 16 planes and 32x7 over 32 planes with two barriers
+//       per plane (double-buffered plane tile).
+constexpr int NSHAPES1 = 2, NSHAPES2 = 5, NSHAPES3 = 6;
 constexpr int SHAPE1_TX[NSHAPES1] = {256, 64};
 constexpr int SHAPE2_TX[NSHAPES2] = {32, 16, 64, 32, 24};
 constexpr int SHAPE2_TY[NSHAPES2] = {16, 15, 4, 15, 20};
-constexpr int SHAPE3_TX[NSHAPES3] = {32, 16, 32};
-constexpr int SHAPE3_TY[NSHAPES3] = {8, 15, 7};
+constexpr int SHAPE3_TX[NSHAPES3] = {32, 16, 32, 16, 16, 32};
+constexpr int SHAPE3_TY[NSHAPES3] = {8, 15, 7, 15, 15, 7};
+constexpr int SHAPE3_LZ[NSHAPES3] = {32, 32, 32, 32, 16, 32};
+constexpr bool SHAPE3_TWO_BARRIERS[NSHAPES3] = {false, false, false, true, true, true};
 
 inline int num_shapes(int dim) { return dim == 1 ? NSHAPES1 : (dim == 2 ? NSHAPES2 : NSHAPES3); }
 
@@ -101,11 +105,19 @@ template <class EQ, int RECON, class FLUX, int DIM>
 hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
     if constexpr (DIM == 3) {
         typedef typename FLUX::Plain F3;  // compiler sqrt: the 3D kernel's code is large
+#define AVH_MARCH_CASE(S)                                                                                   \
+    case S:                                                                                                 \
+        return launch_march_k<MarchKernel3<EQ, RECON, F3, SHAPE3_TX[S], SHAPE3_TY[S], SHAPE3_LZ[S], 256,   \
+                                           SHAPE3_TWO_BARRIERS[S]>>(P, stream);
         switch (shape) {
-        case 0: return launch_march_k<MarchKernel3<EQ, RECON, F3, SHAPE3_TX[0], SHAPE3_TY[0], 32, 256>>(P, stream);
-        case 1: return launch_march_k<MarchKernel3<EQ, RECON, F3, SHAPE3_TX[1], SHAPE3_TY[1], 32, 256>>(P, stream);
-        case 2: return launch_march_k<MarchKernel3<EQ, RECON, F3, SHAPE3_TX[2], SHAPE3_TY[2], 32, 256>>(P, stream);
+            AVH_MARCH_CASE(0)
+            AVH_MARCH_CASE(1)
+            AVH_MARCH_CASE(2)
+            AVH_MARCH_CASE(3)
+            AVH_MARCH_CASE(4)
+            AVH_MARCH_CASE(5)
         }
+#undef AVH_MARCH_CASE
         return hipErrorInvalidValue;
     } else if constexpr (DIM == 1) {
         switch (shape) {

@@ -316,6 +316,9 @@ struct avh_batch {
     int graph_steps = 8;  // steps per captured hipGraph (0: ordinary launches)
     StepGraph graph;
     int shape = -1;       // tile shape of the stage kernel; -1: autotune on first use
+    // groups the stage and element-wise launches cover: a group whose
+    // samples have all stopped is dropped until the next avh_batch_advance
+    bool live[MAX_GROUPS] = {true, true, true, true};
 };
 
 namespace {
@@ -344,17 +347,25 @@ void free_batch(avh_batch* b) {
     delete b;
 }
 
-BatchParams batch_params(const avh_batch* b, int mode, bool ew_blocks, unsigned int* nblocks) {
+// Launch parameters over the live groups (slot j of P is group index[j]);
+// nblocks = the launch's block count (8 * per_xcd for a stage launch, the
+// element-wise blocks of every sample otherwise), 0 if no group is live.
+BatchParams batch_params(const avh_batch* b, int mode, bool ew_blocks, unsigned int* nblocks,
+                         int* index = nullptr) {
     BatchParams P;
     std::memset(&P, 0, sizeof P);
-    P.ngroups = b->ngroups;
     P.bc = b->cfg.boundary;
     P.mode = mode;
     P.gamma = b->cfg.gamma;
     unsigned int first = 0;
+    unsigned int per_xcd[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int j = 0;
     for (int i = 0; i < b->ngroups; ++i) {
+        if (!b->live[i]) continue;
         const GroupHost& G = b->g[i];
-        GroupDesc& D = P.g[i];
+        GroupDesc& D = P.g[j];
+        if (index) index[j] = i;
+        ++j;
         D.ctrl = G.ctrl;
         D.stride = (long long)b->nv * G.ncells;
         D.ncells = G.ncells;
@@ -363,8 +374,12 @@ BatchParams batch_params(const avh_batch* b, int mode, bool ew_blocks, unsigned 
         D.first_block = first;
         D.nsamples = G.nsamples;
         first += D.tiles * (unsigned int)G.nsamples;
+        for (unsigned int x = 0; x < 8; ++x) per_xcd[x] += xcd_samples(G.nsamples, x) * D.tiles;
     }
-    *nblocks = first;
+    P.ngroups = j;
+    P.per_xcd = 0;
+    for (unsigned int x = 0; x < 8; ++x) P.per_xcd = std::max(P.per_xcd, per_xcd[x]);
+    *nblocks = ew_blocks ? first : 8u * P.per_xcd;
     return P;
 }
 
@@ -411,8 +426,11 @@ TuneKey batch_tune_key(const avh_batch* b) {
 // Times every tile shape with flux-only launches over all samples (they write
 // the L0 buffers and add to the per-sample wave-speed maxima; the control
 // blocks are restored afterwards) and keeps the fastest.
+void set_all_live(avh_batch* b);
+
 avh_status batch_autotune(avh_batch* b, int* best, double* ms, int* nshapes) {
     const int ns = num_shapes(b->dim);
+    set_all_live(b);  // time the launches over every group
     AVH_HIP(hipStreamSynchronize(b->stream));
     std::vector<std::vector<Control>> saved(b->ngroups);
     for (int i = 0; i < b->ngroups; ++i) {
@@ -475,48 +493,64 @@ int batch_launches_per_step(const avh_batch* b) {
     return stages + 1 + (b->nsaves > 0 ? 1 : 0);
 }
 
+void set_all_live(avh_batch* b) {
+    bool changed = false;
+    for (int i = 0; i < b->ngroups; ++i) {
+        changed = changed || !b->live[i];
+        b->live[i] = true;
+    }
+    if (changed) b->graph.reset();
+}
+
 avh_status enqueue_batch_step(avh_batch* b, EventMarks* tm = nullptr) {
     avh_status r;
     if ((r = mark(tm, b->stream))) return r;
     unsigned int nb = 0, nbe = 0;
+    int index[MAX_GROUPS] = {0, 0, 0, 0};
     auto set_ptrs = [&](BatchParams& P, int which_in, int which_u0, int which_out) {
-        for (int i = 0; i < b->ngroups; ++i) {
-            double* bufs[3] = {b->g[i].A, b->g[i].B, b->g[i].C};
-            P.g[i].in = bufs[which_in];
-            P.g[i].u0 = bufs[which_u0];
-            P.g[i].out = bufs[which_out];
+        for (int j = 0; j < P.ngroups; ++j) {
+            const GroupHost& G = b->g[index[j]];
+            double* bufs[3] = {G.A, G.B, G.C};
+            P.g[j].in = bufs[which_in];
+            P.g[j].u0 = bufs[which_u0];
+            P.g[j].out = bufs[which_out];
         }
     };
+    // a launch with no live group is left out (every sample has stopped;
+    // the end-of-step kernel then leaves them unchanged)
+    auto stage = [&](BatchParams& P) -> avh_status {
+        if (nb) AVH_HIP(b->stage(P, nb, b->shape, b->stream));
+        return mark(tm, b->stream);
+    };
+    auto ew = [&](BatchParams& E, int fe_finish) -> avh_status {
+        if (nbe) {
+            b->ew(E, nbe, fe_finish, b->stream);
+            AVH_HIP(hipGetLastError());
+        }
+        return mark(tm, b->stream);
+    };
     enum { A = 0, B = 1, C = 2 };
-    BatchParams P = batch_params(b, MODE_L, false, &nb);
+    BatchParams P = batch_params(b, MODE_L, false, &nb, index);
     set_ptrs(P, A, A, B);
-    AVH_HIP(b->stage(P, nb, b->shape, b->stream));
-    if ((r = mark(tm, b->stream))) return r;
+    if ((r = stage(P))) return r;
     if (b->cfg.integrator == AVH_INT_FORWARDEULER) {
-        BatchParams E = batch_params(b, MODE_L, true, &nbe);
+        BatchParams E = batch_params(b, MODE_L, true, &nbe, index);
         set_ptrs(E, B, A, A);
-        b->ew(E, nbe, 1, b->stream);
-        AVH_HIP(hipGetLastError());
-        if ((r = mark(tm, b->stream))) return r;
+        if ((r = ew(E, 1))) return r;
     } else if (b->cfg.integrator == AVH_INT_RUNGEKUTTA2) {
-        BatchParams E = batch_params(b, MODE_L, true, &nbe);
+        BatchParams E = batch_params(b, MODE_L, true, &nbe, index);
         set_ptrs(E, B, A, C);
-        b->ew(E, nbe, 0, b->stream);
-        AVH_HIP(hipGetLastError());
-        if ((r = mark(tm, b->stream))) return r;
-        P = batch_params(b, MODE_RK2_2, false, &nb);
+        if ((r = ew(E, 0))) return r;
+        P = batch_params(b, MODE_RK2_2, false, &nb, index);
         set_ptrs(P, C, A, A);
-        AVH_HIP(b->stage(P, nb, b->shape, b->stream));
-        if ((r = mark(tm, b->stream))) return r;
+        if ((r = stage(P))) return r;
     } else {
-        P = batch_params(b, MODE_RK3_2, false, &nb);
+        P = batch_params(b, MODE_RK3_2, false, &nb, index);
         set_ptrs(P, B, A, C);
-        AVH_HIP(b->stage(P, nb, b->shape, b->stream));
-        if ((r = mark(tm, b->stream))) return r;
-        P = batch_params(b, MODE_RK3_3, false, &nb);
+        if ((r = stage(P))) return r;
+        P = batch_params(b, MODE_RK3_3, false, &nb, index);
         set_ptrs(P, C, A, A);
-        AVH_HIP(b->stage(P, nb, b->shape, b->stream));
-        if ((r = mark(tm, b->stream))) return r;
+        if ((r = stage(P))) return r;
     }
     unsigned int nbs = 0;
     StatsParams S = stats_params(b, &nbs);
@@ -744,6 +778,10 @@ avh_status avh_batch_advance(avh_batch* b, int64_t max_steps, int chunk, int64_t
     }
     avh_status rs = batch_ensure_shape(b);
     if (rs) return rs;
+    // every group takes part until a chunk shows that all its samples have
+    // stopped (the end-of-step kernel counts the running samples of every
+    // group, live or not, at every step)
+    set_all_live(b);
     int64_t done = 0, left = 0;
     while (done < max_steps) {
         const int64_t k = std::min<int64_t>(chunk, max_steps - done);
@@ -757,6 +795,10 @@ avh_status avh_batch_advance(avh_batch* b, int64_t max_steps, int chunk, int64_t
             int a = 0;
             AVH_HIP(hipMemcpy(&a, b->g[i].active_count, sizeof(int), hipMemcpyDeviceToHost));
             left += a;
+            if (a == 0 && b->live[i]) {
+                b->live[i] = false;  // drop the finished group from the launches
+                b->graph.reset();
+            }
         }
         if (left == 0) break;
     }

@@ -16,7 +16,12 @@ no HDF5 library exists on this platform, so the files are classic NetCDF
 (64-bit offset), which every NetCDF reader, including netCDF4-python and
 xarray, opens the same way.
 """
+import getpass
 import os
+import platform
+import socket
+import sys
+import time as _time
 
 import numpy as np
 from scipy.io import netcdf_file
@@ -29,8 +34,47 @@ CONSERVED = {
 }
 
 
-def write_snapshot(filename, variables, time, attributes=None):
-    """variables: {name: interior array of shape (nz, ny, nx)} (x fastest)."""
+def _cpu_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def basic_report():
+    """The run report every reference NetCDF file carries as global text
+    attributes `alsvinn_report.<key>` (alsfvm/src/io/netcdf_write_report.cpp:25-29,
+    the keys of alsutils/src/make_basic_report.cpp:46-90), with this
+    platform's values."""
+    try:
+        user = getpass.getuser()
+    except Exception:
+        user = "unknown"
+    libc = " ".join(platform.libc_ver()).strip() or "unknown"
+    report = {
+        "software": "alsvinn_amd (alsvinn hip platform)",
+        "softwareVersion": "hip",
+        "currentWorkingDirectory": os.getcwd(),
+        "operatingSystem": "%s %s" % (platform.system(), platform.release()),
+        "username": user,
+        "host": socket.gethostname(),
+        "standardCLibrary": libc,
+        "generatedAt": _time.strftime("%Y%m%dT%H%M%S", _time.localtime()),
+        "CPU": _cpu_name(),
+        "platform": "hip",
+        "pythonVersion": sys.version.split()[0],
+    }
+    return {"alsvinn_report." + k: str(v) for k, v in report.items()}
+
+
+def write_snapshot(filename, variables, time, attributes=None, report=True):
+    """variables: {name: interior array of shape (nz, ny, nx)} (x fastest).
+    The run report's attributes come first, then `attributes`, as the
+    reference writes them."""
     arrs = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in variables.items()}
     shapes = {a.shape for a in arrs.values()}
     if len(shapes) != 1:
@@ -38,6 +82,8 @@ def write_snapshot(filename, variables, time, attributes=None):
     nz, ny, nx = next(iter(shapes))
     f = netcdf_file(filename, "w", version=2)
     try:
+        for k, v in (basic_report() if report else {}).items():
+            setattr(f, k, v)
         for k, v in (attributes or {}).items():
             setattr(f, k, v)
         f.createDimension("t", 1)

@@ -27,7 +27,9 @@ import time
 
 HBM_PEAK_GBS = 8000.0
 SHAPE_NAMES = {1: ["256", "64"], 2: ["32x16", "16x15", "64x4", "32x15", "24x20"],
-               3: ["32x8 columns x 32 planes", "16x15 columns x 32 planes", "32x7 columns x 32 planes"]}
+               3: ["32x8 columns x 32 planes", "16x15 columns x 32 planes", "32x7 columns x 32 planes",
+                   "16x15 columns x 32 planes, 2 barriers", "16x15 columns x 16 planes, 2 barriers",
+                   "32x7 columns x 32 planes, 2 barriers"]}
 
 WORKLOADS = {
     # the bench line

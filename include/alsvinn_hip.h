@@ -156,6 +156,24 @@ avh_status avh_synchronize(avh_solver* solver);
  * once and replayed; results are identical either way.  The reference
  * launches each kernel separately (alsfvm/src/simulator/Simulator.cpp:191-214). */
 avh_status avh_set_graph_steps(avh_solver* solver, int steps);
+/* One full time step of cfg's scheme (all RK stages, CFL dt, constraint
+ * check) on a device state, allocating nothing: `state` is the interior-only
+ * SoA (nvars * n[0]*n[1]*n[2] doubles, x fastest) and is advanced in place;
+ * dt = min(cfl / sum_d(ws_d / dx_d), dt_cap) (dt_cap = +inf: no clip);
+ * dt_out[0] (device double, may be NULL) receives dt and violated_out[0]
+ * (device int, may be NULL) 1 if the new state breaks the constraints.
+ * Everything is enqueued on `stream` (NULL: the null stream) with no host
+ * synchronisation, no memset and no allocation, so the call can be
+ * captured into a hipGraph.  The stage buffers live in a static device
+ * array of 2 x 4194304 doubles (64 MiB, held by every GPU the library runs
+ * on): states of more than 4194304 values (nvars * cells) are rejected with
+ * AVH_EINVAL.  All calls on one GPU share that array: calls that overlap in
+ * time on one GPU (other streams, other threads) race.  The tile shape is
+ * the autotuner's choice if a solver of the same problem made one, else a
+ * fixed default.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops). */
+avh_status avh_step_noalloc(avh_ctx* ctx, const avh_config* cfg, double* state, double dt_cap, double* dt_out,
+                            int* violated_out, void* stream);
 /* Resolve the tile shape (autotuning if none is fixed) and capture and
  * instantiate the step-chunk graph now, without launching it, so that a
  * later avh_enqueue_steps / avh_advance only replays.  No reference

@@ -109,3 +109,42 @@ def test_netcdf_writer_numbers_snapshots(tmp_path):
     w = NetCDFWriter(str(tmp_path / "sub" / "run"))
     names = [w.write({"u": np.zeros((1, 1, 5))}, t) for t in (0.0, 0.5)]
     assert [os.path.basename(n) for n in names] == ["run_0.nc", "run_1.nc"]
+
+
+def test_cli_decomposition_flags_accepted_and_checked():
+    """alsvinncli / alsuqcli accept the reference's MPI options
+    (alsvinncli/src/main.cpp:55-67, alsuqcli/src/main.cpp:40-51); one GPU
+    means every count must be 1."""
+    import argparse
+    from alsvinn_amd.alsvinncli import add_decomposition_flags, check_decomposition
+    for sample in (False, True):
+        ap = argparse.ArgumentParser()
+        ap.add_argument("input")
+        add_decomposition_flags(ap, sample=sample)
+        args = ["cfg.xml", "--multi-x", "1", "--multi-y", "1", "--multi-z", "1"]
+        if sample:
+            args += ["--multi-sample", "1"]
+        else:
+            args += ["-x"]
+        a = ap.parse_args(args)
+        check_decomposition(ap, a, sample=sample)
+        bad = ["cfg.xml", "--multi-sample" if sample else "--multi-y", "2"]
+        with pytest.raises(SystemExit):
+            check_decomposition(ap, ap.parse_args(bad), sample=sample)
+
+
+def test_netcdf_files_carry_the_run_report(tmp_path):
+    """alsvinn_report.* global attributes (alsfvm/src/io/netcdf_write_report.cpp:25-29),
+    then the writer's own attributes."""
+    from scipy.io import netcdf_file
+    from alsvinn_amd.netcdf import NetCDFWriter
+    w = NetCDFWriter(str(tmp_path / "kh"), {"fvm_name": "kh"})
+    fn = w.write({"rho": np.ones((1, 3, 4))}, 0.25)
+    f = netcdf_file(fn, "r", mmap=False)
+    try:
+        attrs = {k: (v.decode() if isinstance(v, bytes) else v) for k, v in f._attributes.items()}
+    finally:
+        f.close()
+    for key in ("software", "host", "CPU", "generatedAt", "currentWorkingDirectory", "operatingSystem"):
+        assert attrs["alsvinn_report." + key]
+    assert attrs["fvm_name"] == "kh"
