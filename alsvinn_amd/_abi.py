@@ -77,6 +77,7 @@ SYMBOLS = [
     "avh_mlp_get_params",
     "avh_mlp_set_params",
     "avh_mlp_gradients",
+    "avh_mlp_set_weight_decay",
     "avh_mlp_adam_step",
     "avh_mlp_train",
     "avh_mlp_predict",
@@ -229,6 +230,7 @@ def load():
         "avh_mlp_set_params": [vp, ctypes.c_int, vp],
         "avh_mlp_gradients": [vp, vp, vp, i64, vp, vp],
         "avh_mlp_adam_step": [vp, ctypes.c_float],
+        "avh_mlp_set_weight_decay": [vp, ctypes.c_float],
         "avh_mlp_train": [vp, vp, vp, i64, ctypes.c_int, ctypes.c_float, ctypes.c_float, vp],
         "avh_mlp_predict": [vp, vp, i64, vp],
     }

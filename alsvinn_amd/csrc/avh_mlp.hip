@@ -194,10 +194,13 @@ struct AdamArgs {
     long long p_ms, wb_ms;
     LayerDesc L[MAX_LAYERS];
     float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
+    float wd;                       // decoupled weight decay of the weights (not the biases)
 };
 
 // Adam (bias-corrected) on the fp32 master parameters of layer blockIdx.y,
-// then the bf16 copies of the weights in both orientations.
+// with decoupled weight decay on the weights (AdamW, Loshchilov and Hutter,
+// "Decoupled Weight Decay Regularization", 2019: p -= lr * wd * p beside the
+// Adam step), then the bf16 copies of the weights in both orientations.
 __global__ __launch_bounds__(256) void mlp_adam_kernel(AdamArgs a) {
     const LayerDesc& L = a.L[blockIdx.y];
     const int mem = blockIdx.z;
@@ -210,7 +213,9 @@ __global__ __launch_bounds__(256) void mlp_adam_kernel(AdamArgs a) {
     const float v = a.b2 * a.v[i] + (1.f - a.b2) * (g * g);
     a.m[i] = m;
     a.v[i] = v;
-    const float p = a.p[i] - a.lr * (m / a.c1) / (sqrtf(v / a.c2) + a.eps);
+    const float p0 = a.p[i];
+    float p = p0 - a.lr * (m / a.c1) / (sqrtf(v / a.c2) + a.eps);
+    if (e < nw && a.wd != 0.f) p = p - a.lr * a.wd * p0;
     a.p[i] = p;
     if (e < nw) {
         const int o = (int)(e / L.nin), k = (int)(e % L.nin);
@@ -257,6 +262,8 @@ struct avh_mlp {
     int* idx = nullptr;
     long long icap = 0;
     long long t = 0;  // Adam steps taken
+    long long epochs_done = 0;  // epochs of avh_mlp_train so far (seeds the shuffles)
+    float wd = 0.f;   // decoupled weight decay
     hipStream_t stream = nullptr;
 };
 
@@ -445,6 +452,7 @@ avh_status adam_step(avh_mlp* n, float lr) {
     n->t += 1;
     AdamArgs a = adam_args(n);
     a.lr = lr;
+    a.wd = n->wd;
     a.b1 = n->cfg.beta1;
     a.b2 = n->cfg.beta2;
     a.eps = n->cfg.eps;
@@ -698,7 +706,8 @@ avh_status avh_mlp_adam_step(avh_mlp* n, float lr) {
 
 // Minibatch Adam over `epochs` passes of the rows; each member visits the
 // rows in its own seeded order (Fisher-Yates from std::mt19937_64(seed +
-// 1000003 * (epoch + 1) + member)), floor(rows / batch) steps per epoch;
+// 1000003 * (epoch + 1) + member), epochs counted over every call on this
+// network), floor(rows / batch) steps per epoch;
 // learning rate lr * (final_lr_fraction + (1 - final_lr_fraction) * (1 +
 // cos(pi * step / steps)) / 2).  epoch_loss[e][member] = mean minibatch loss.
 avh_status avh_mlp_train(avh_mlp* n, const float* X, const float* Y, int64_t rows, int epochs, float lr,
@@ -717,7 +726,7 @@ avh_status avh_mlp_train(avh_mlp* n, const float* X, const float* Y, int64_t row
     long long k = 0;
     for (int e = 0; e < epochs; ++e) {
         for (int mem = 0; mem < n->M; ++mem) {
-            std::mt19937_64 gen(n->cfg.seed + 1000003ull * (uint64_t)(e + 1) + (uint64_t)mem);
+            std::mt19937_64 gen(n->cfg.seed + 1000003ull * (uint64_t)(n->epochs_done + e + 1) + (uint64_t)mem);
             int* p = perm.data() + (size_t)mem * rows;
             for (long long i = 0; i < rows; ++i) p[i] = (int)i;
             for (long long i = rows - 1; i > 0; --i) {
@@ -740,6 +749,14 @@ avh_status avh_mlp_train(avh_mlp* n, const float* X, const float* Y, int64_t row
         if (epoch_loss)
             for (int mem = 0; mem < n->M; ++mem) epoch_loss[(size_t)e * n->M + mem] = lh[mem] / (float)steps_per_epoch;
     }
+    n->epochs_done += epochs;
+    return AVH_OK;
+}
+
+avh_status avh_mlp_set_weight_decay(avh_mlp* n, float weight_decay) {
+    if (!n) return fail(AVH_EINVAL, "null argument");
+    if (!(weight_decay >= 0.f) || weight_decay > 1.f) return fail(AVH_EINVAL, "weight decay must be in [0, 1]");
+    n->wd = weight_decay;
     return AVH_OK;
 }
 

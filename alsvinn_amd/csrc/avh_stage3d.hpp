@@ -23,8 +23,10 @@ namespace avh {
 
 // ENO / WENO2 / none on a register window: u[i][v], stencil centre `c`
 // (compile-time), unit stride.  Selects replace ENO's data-dependent index.
-template <int RECON, int NV, bool RIGHT, int WZ>
-__device__ __forceinline__ void recon_face_reg(const double (&u)[WZ][NV], const int c, double* out) {
+// OPS as in recon_face_ops (the WENO2 weights may share a reciprocal).
+template <int RECON, int NV, bool RIGHT, int WZ, class OPS = PlainOps>
+__device__ __forceinline__ void recon_face_reg(const double (&u)[WZ][NV], const int c, double* out,
+                                               bool& ok) {
     if (RECON == RECON_NONE) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) out[v] = u[c][v];
@@ -40,16 +42,16 @@ __device__ __forceinline__ void recon_face_reg(const double (&u)[WZ][NV], const 
         if (RIGHT) {
             const double a0 = 2 / d0;
             const double a1 = 1 / d1;
-            const double w0r = a0 / (a0 + a1);
-            const double w1r = a1 / (a0 + a1);
+            double w0r, w1r;
+            weno2_weights<OPS>(a0, a1, w0r, w1r, ok);
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 out[v] = 0.5 * (w0r * u[c + 1][v] + (3 * w1r + w0r) * u[c][v] - w1r * u[c - 1][v]);
         } else {
             const double a0 = 1 / d0;
             const double a1 = 2 / d1;
-            const double w0l = a0 / (a0 + a1);
-            const double w1l = a1 / (a0 + a1);
+            double w0l, w1l;
+            weno2_weights<OPS>(a0, a1, w0l, w1l, ok);
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 out[v] = 0.5 * (w1l * u[c - 1][v] + (3 * w0l + w1l) * u[c][v] - w0l * u[c + 1][v]);
@@ -88,6 +90,12 @@ __device__ __forceinline__ void recon_face_reg(const double (&u)[WZ][NV], const 
             out[v] = acc;
         }
     }
+}
+
+template <int RECON, int NV, bool RIGHT, int WZ>
+__device__ __forceinline__ void recon_face_reg(const double (&u)[WZ][NV], const int c, double* out) {
+    bool ok = true;
+    recon_face_reg<RECON, NV, RIGHT, WZ, PlainOps>(u, c, out, ok);
 }
 
 // TWO_BARRIERS: the plane tile is double-buffered and the x- and y-interface
@@ -129,33 +137,66 @@ struct MarchKernel3 {
         }
     }
 
+    // interface between plane cells ia and ia + SD under OPS
+    template <class OPS, int DIR>
+    __device__ __forceinline__ static double plane_interface(const double* sp, int ia, int SD, double gamma,
+                                                             double* F, bool& ok) {
+        double qa[NV], qb[NV];
+        recon_face_ops<RECON, NV, true, OPS>(sp, LP, ia, SD, qa, ok);
+        recon_face_ops<RECON, NV, false, OPS>(sp, LP, ia + SD, SD, qb, ok);
+        const typename EQ::All aL = EQ::template make_all_ops<OPS>(qa, gamma, ok);
+        const typename EQ::All aR = EQ::template make_all_ops<OPS>(qb, gamma, ok);
+        return FLUX::template flux_ops<OPS, DIR, EQ>(aL, aR, gamma, F, ok);
+    }
+
+    template <int DIR>
+    __device__ __forceinline__ static bool plane_item(int it, int ox, int oy, const int* n, int& ia) {
+        int cx, cy;
+        if (DIR == 0) {
+            cx = it % (TX + 1) - 1;
+            cy = it / (TX + 1);
+        } else {
+            cx = it % TX;
+            cy = it / TX - 1;
+        }
+        ia = (cy + G) * LXP + (cx + G);
+        return (ox + cx < n[0]) && (oy + cy < n[1]);
+    }
+
+    // A FAST flux evaluates each interface with SharedOps and redoes those
+    // whose operands leave the guarded range with PlainOps (as the 2D stage
+    // kernel's flux_pass); otherwise PlainOps throughout.
     template <int DIR>
     __device__ __forceinline__ static void plane_pass(const double* sp, double* sf, int ox, int oy, const int* n,
                                                       double gamma, double& ws) {
         constexpr int NI = DIR == 0 ? NIX : NIY;
         constexpr int SD = DIR == 0 ? 1 : LXP;
-        for (int it = threadIdx.x; it < NI; it += NT) {
-            int cx, cy;
-            if (DIR == 0) {
-                cx = it % (TX + 1) - 1;
-                cy = it / (TX + 1);
-            } else {
-                cx = it % TX;
-                cy = it / TX - 1;
-            }
-            const bool need = (ox + cx < n[0]) && (oy + cy < n[1]);
-            if (!need) continue;
-            const int ia = (cy + G) * LXP + (cx + G);
-            double qa[NV], qb[NV];
-            recon_face<RECON, NV, true>(sp, LP, ia, SD, qa);
-            recon_face<RECON, NV, false>(sp, LP, ia + SD, SD, qb);
-            const typename EQ::All aL = EQ::make_all(qa, gamma);
-            const typename EQ::All aR = EQ::make_all(qb, gamma);
+        static_assert((NI + NT - 1) / NT <= 32, "one redo bit per turn");
+        typedef typename OpsFor<FLUX::FAST>::type Ops;
+        unsigned int redo = 0;
+        int m = 0;
+        for (int it = threadIdx.x; it < NI; it += NT, ++m) {
+            int ia;
+            if (!plane_item<DIR>(it, ox, oy, n, ia)) continue;
             double F[NV];
-            const double w = FLUX::template flux<DIR, EQ>(aL, aR, gamma, F);
-            ws = ws_max(ws, w);
+            bool ok = true;
+            const double w = plane_interface<Ops, DIR>(sp, ia, SD, gamma, F, ok);
+            if (ok) ws = ws_max(ws, w);
+            else redo |= 1u << m;
 #pragma unroll
             for (int v = 0; v < NV; ++v) sf[v * NIMAX + it] = F[v];
+        }
+        if (FLUX::FAST && redo != 0) {
+            m = 0;
+            for (int it = threadIdx.x; it < NI; it += NT, ++m) {
+                int ia;
+                if (!((redo >> m) & 1u) || !plane_item<DIR>(it, ox, oy, n, ia)) continue;
+                double F[NV];
+                bool ok = true;
+                ws = ws_max(ws, plane_interface<PlainOps, DIR>(sp, ia, SD, gamma, F, ok));
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sf[v * NIMAX + it] = F[v];
+            }
         }
     }
 
@@ -339,12 +380,23 @@ struct MarchKernel3 {
             for (int v = 0; v < NV; ++v) fz[v] = rnext[v] = 0.0;
             auto z_flux = [&]() {
                 if (has_col) {
+                    typedef typename OpsFor<FLUX::FAST>::type Ops;
                     double lz[NV];
-                    recon_face_reg<RECON, NV, false, WZ>(w, HW, lz);
-                    recon_face_reg<RECON, NV, true, WZ>(w, HW, rnext);
-                    const typename EQ::All aL = EQ::make_all(rprev, P.gamma);
-                    const typename EQ::All aR = EQ::make_all(lz, P.gamma);
-                    const double wz = FLUX::template flux<2, EQ>(aL, aR, P.gamma, fz);
+                    bool ok = true;
+                    recon_face_reg<RECON, NV, false, WZ, Ops>(w, HW, lz, ok);
+                    recon_face_reg<RECON, NV, true, WZ, Ops>(w, HW, rnext, ok);
+                    typename EQ::All aL = EQ::template make_all_ops<Ops>(rprev, P.gamma, ok);
+                    typename EQ::All aR = EQ::template make_all_ops<Ops>(lz, P.gamma, ok);
+                    double wz = FLUX::template flux_ops<Ops, 2, EQ>(aL, aR, P.gamma, fz, ok);
+                    if (FLUX::FAST && !ok) {
+                        // operands outside the guarded range: the plain path
+                        recon_face_reg<RECON, NV, false, WZ>(w, HW, lz);
+                        recon_face_reg<RECON, NV, true, WZ>(w, HW, rnext);
+                        aL = EQ::make_all(rprev, P.gamma);
+                        aR = EQ::make_all(lz, P.gamma);
+                        bool ok2 = true;
+                        wz = FLUX::template flux_ops<PlainOps, 2, EQ>(aL, aR, P.gamma, fz, ok2);
+                    }
                     if (col_ok) ws[2] = ws_max(ws[2], wz);
                 }
             };

@@ -31,15 +31,17 @@ typedef hipError_t (*VolumeLauncher)(const StageParams& P, const VolumeParams& V
 //       plane interfaces), 16x15 or 32x7 (at most 256 each: one turn; the
 //       threads past the columns only work on the planes); then 16x15 over
 //       32 and over 16 planes and 32x7 over 32 planes with two barriers
-//       per plane (double-buffered plane tile).
-constexpr int NSHAPES1 = 2, NSHAPES2 = 5, NSHAPES3 = 6;
+//       per plane (double-buffered plane tile); then 16x15 over 32 planes,
+//       two barriers, fluxes under SharedOps with the PlainOps redo.
+constexpr int NSHAPES1 = 2, NSHAPES2 = 5, NSHAPES3 = 7;
 constexpr int SHAPE1_TX[NSHAPES1] = {256, 64};
 constexpr int SHAPE2_TX[NSHAPES2] = {32, 16, 64, 32, 24};
 constexpr int SHAPE2_TY[NSHAPES2] = {16, 15, 4, 15, 20};
-constexpr int SHAPE3_TX[NSHAPES3] = {32, 16, 32, 16, 16, 32};
-constexpr int SHAPE3_TY[NSHAPES3] = {8, 15, 7, 15, 15, 7};
-constexpr int SHAPE3_LZ[NSHAPES3] = {32, 32, 32, 32, 16, 32};
-constexpr bool SHAPE3_TWO_BARRIERS[NSHAPES3] = {false, false, false, true, true, true};
+constexpr int SHAPE3_TX[NSHAPES3] = {32, 16, 32, 16, 16, 32, 16};
+constexpr int SHAPE3_TY[NSHAPES3] = {8, 15, 7, 15, 15, 7, 15};
+constexpr int SHAPE3_LZ[NSHAPES3] = {32, 32, 32, 32, 16, 32, 32};
+constexpr bool SHAPE3_TWO_BARRIERS[NSHAPES3] = {false, false, false, true, true, true, true};
+constexpr bool SHAPE3_SHARED[NSHAPES3] = {false, false, false, false, false, false, true};
 
 inline int num_shapes(int dim) { return dim == 1 ? NSHAPES1 : (dim == 2 ? NSHAPES2 : NSHAPES3); }
 
@@ -104,10 +106,13 @@ hipError_t launch_march_k(const StageParams& P, hipStream_t stream) {
 template <class EQ, int RECON, class FLUX, int DIM>
 hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
     if constexpr (DIM == 3) {
-        typedef typename FLUX::Plain F3;  // compiler sqrt: the 3D kernel's code is large
+        // shapes without SharedOps use the compiler's sqrt and divide throughout
 #define AVH_MARCH_CASE(S)                                                                                   \
     case S:                                                                                                 \
-        return launch_march_k<MarchKernel3<EQ, RECON, F3, SHAPE3_TX[S], SHAPE3_TY[S], SHAPE3_LZ[S], 256,   \
+        return launch_march_k<MarchKernel3<EQ, RECON,                                                       \
+                                           typename std::conditional<SHAPE3_SHARED[S], FLUX,                \
+                                                                     typename FLUX::Plain>::type,           \
+                                           SHAPE3_TX[S], SHAPE3_TY[S], SHAPE3_LZ[S], 256,                   \
                                            SHAPE3_TWO_BARRIERS[S]>>(P, stream);
         switch (shape) {
             AVH_MARCH_CASE(0)
@@ -116,6 +121,7 @@ hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
             AVH_MARCH_CASE(3)
             AVH_MARCH_CASE(4)
             AVH_MARCH_CASE(5)
+            AVH_MARCH_CASE(6)
         }
 #undef AVH_MARCH_CASE
         return hipErrorInvalidValue;

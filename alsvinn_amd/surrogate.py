@@ -94,6 +94,10 @@ class EnsembleMLP:
     def adam_step(self, lr):
         _abi.check(_abi.load().avh_mlp_adam_step(self._h, float(lr)))
 
+    def set_weight_decay(self, weight_decay):
+        """Decoupled weight decay (AdamW) of the weights in later steps."""
+        _abi.check(_abi.load().avh_mlp_set_weight_decay(self._h, float(weight_decay)))
+
     def train(self, X, Y, epochs, lr=1e-3, final_lr_fraction=0.05):
         """Returns the mean minibatch loss per epoch and member."""
         X = self._f32(X)
@@ -104,6 +108,40 @@ class EnsembleMLP:
         _abi.check(_abi.load().avh_mlp_train(self._h, X.ctypes.data, Y.ctypes.data, rows, int(epochs), float(lr),
                                              float(final_lr_fraction), hist.ctypes.data))
         return hist
+
+    def train_early_stopping(self, X, Y, X_val, Y_val, max_epochs=300, check_every=10, lr=1e-3, patience=5):
+        """Adam at a constant learning rate in rounds of `check_every` epochs;
+        after each round every member's mean squared error on the validation
+        rows (in units of the outputs' standard deviations) is measured, and
+        each member keeps the parameters of its best round.  Training stops
+        when no member improved for `patience` rounds.  Returns the
+        validation error per round and member, and the best round's per
+        member."""
+        Xv = self._f32(X_val)
+        Yv = np.asarray(Y_val, dtype=np.float64)
+        sd = np.maximum(Yv.std(0), 1e-300)
+        best = [None] * self.members
+        best_err = np.full(self.members, np.inf)
+        best_round = np.zeros(self.members, dtype=int)
+        hist = []
+        stale = 0
+        for rnd in range(max(1, max_epochs // check_every)):
+            self.train(X, Y, epochs=check_every, lr=lr, final_lr_fraction=1.0)
+            pm = self.predict_members(Xv).astype(np.float64)
+            err = np.mean(((pm - Yv[None]) / sd) ** 2, axis=(1, 2))
+            hist.append(err)
+            improved = err < best_err
+            for m in np.nonzero(improved)[0]:
+                best[m] = self.get_params(int(m))
+                best_err[m] = err[m]
+                best_round[m] = rnd
+            stale = 0 if improved.any() else stale + 1
+            if stale >= patience:
+                break
+        for m in range(self.members):
+            if best[m] is not None:
+                self.set_params(m, best[m])
+        return np.array(hist), best_round
 
     def predict_members(self, X):
         X = self._f32(X)
@@ -167,8 +205,12 @@ def surrogate_statistics(model, X_query, solver=None, threshold=0.05, X_solved=N
                (`solver(X) -> Y`) elsewhere; mean and population variance
                over all query samples.
     control variate (when solver samples X_solved, Y_solved are given):
-               E[f] = E_query[g] + mean over solved samples of (f - g),
-               E[f^2] likewise, var = E[f^2] - E[f]^2; unbiased for any g.
+               E[f] = mean_S(f) - beta (mean_S(g) - mean_Q(g)) with the
+               variance-minimising beta = cov_S(f, g) / var_S(g) per
+               observable (S: solved samples, Q: query samples), E[f^2]
+               likewise with f^2, g^2; var = E[f^2] - E[f]^2.  Its
+               variance is var(f) (1 - rho^2) / |S|, never above plain
+               Monte Carlo's on the same solved samples.
 
     Returns a dict with both estimates (where available), the fallback
     fraction and the wall time of each part."""
@@ -189,20 +231,34 @@ def surrogate_statistics(model, X_query, solver=None, threshold=0.05, X_solved=N
     if X_solved is not None:
         gs, _ = model.predict(X_solved)
         f = np.asarray(Y_solved, dtype=np.float64)
-        m1 = g.mean(0) + (f - gs).mean(0)
-        m2 = (g * g).mean(0) + (f * f - gs * gs).mean(0)
+        b1 = optimal_beta(f, gs)
+        b2 = optimal_beta(f * f, gs * gs)
+        m1 = f.mean(0) - b1 * (gs.mean(0) - g.mean(0))
+        m2 = (f * f).mean(0) - b2 * ((gs * gs).mean(0) - (g * g).mean(0))
+        res["cv_beta"] = b1
         res["cv_mean"] = m1
         res["cv_var"] = m2 - m1 * m1
     return res
 
 
+def optimal_beta(f, g):
+    """cov(f, g) / var(g) per column (0 where g is constant)."""
+    fc = f - f.mean(0)
+    gc = g - g.mean(0)
+    vg = (gc * gc).mean(0)
+    return np.where(vg > 0, (fc * gc).mean(0) / np.where(vg > 0, vg, 1.0), 0.0)
+
+
 def control_variate_efficiency(model, X_solved, Y_solved):
-    """Per observable var(f) / var(f - g) on solver samples the network did
-    not train on: the factor by which the control-variate estimator cuts the
-    number of solver samples for a given error (1 = no gain)."""
+    """Per observable var(f) / var(f - beta g) = 1 / (1 - rho(f, g)^2) on
+    solver samples the network did not train on, beta the variance-minimising
+    coefficient: the factor by which the control-variate estimator cuts the
+    number of solver samples for a given error of the mean (1 = no gain; it
+    is never below 1)."""
     g, _ = model.predict(X_solved)
     f = np.asarray(Y_solved, dtype=np.float64)
-    return f.var(0) / np.maximum((f - g).var(0), 1e-300)
+    b = optimal_beta(f, g)
+    return f.var(0) / np.maximum((f - b * g).var(0), 1e-300)
 
 
 def _rel_err(est, ref):
@@ -224,7 +280,7 @@ def plain_mc_error(pool, ref_mean, ref_var, nsamples, reps=16, seed=0):
 
 def uq_experiment(n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), number_of_h=63, n_train=4096, n_holdout=512,
                   n_ref=16384, n_query=16384, epochs=300, lr=1e-3, members=5, width=256, hidden=4, batch=256,
-                  threshold=0.05, chunk=2048, data=None, log=None, seed=0):
+                  threshold=0.05, chunk=2048, data=None, log=None, seed=0, weight_decay=0.01, check_every=10):
     """Surrogate-assisted UQ of the Kelvin-Helmholtz observables against plain
     Monte Carlo.  Sample ids: training [0, n_train), held out next, then the
     plain-solver reference, then the surrogate's query samples.  `data` may
@@ -256,10 +312,16 @@ def uq_experiment(n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), number_of_h=6
 
     model = EnsembleMLP(40, n_out, width=width, hidden=hidden, members=members, batch=batch, seed=seed)
     model.fit_normalization(Xtr, Ytr)
+    model.set_weight_decay(weight_decay)
+    # early stopping on the last eighth of the training samples (the held-out
+    # samples are kept for the control variate and the error estimates)
+    nv = max(batch, n_train // 8)
     t0 = time.perf_counter()
-    hist = model.train(Xtr, Ytr, epochs=epochs, lr=lr)
+    vhist, best_round = model.train_early_stopping(Xtr[:-nv], Ytr[:-nv], Xtr[-nv:], Ytr[-nv:], max_epochs=epochs,
+                                                   check_every=check_every, lr=lr)
     t_train = time.perf_counter() - t0
-    log("trained %d epochs in %.1f s, loss %.3g -> %.3g" % (epochs, t_train, hist[0].mean(), hist[-1].mean()))
+    log("trained %d rounds of %d epochs in %.1f s, validation error %.3g -> best %.3g (rounds %s)"
+        % (len(vhist), check_every, t_train, vhist[0].mean(), vhist.min(0).mean(), list(best_round)))
     g_ho, s_ho = model.predict(Xho)
     holdout_nmse = (np.mean((g_ho - Yho) ** 2, axis=0) / np.maximum(Yho.var(0), 1e-300))
 
@@ -274,8 +336,10 @@ def uq_experiment(n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), number_of_h=6
     st = surrogate_statistics(model, Xq, solver=solver, threshold=threshold, X_solved=Xho, Y_solved=Yho)
     res = {
         "n": n, "end_time": end_time, "n_train": n_train, "n_holdout": n_holdout, "n_ref": n_ref,
-        "n_query": n_query, "epochs": epochs, "solver_seconds_per_sample": t_sample, "train_seconds": t_train,
-        "final_loss": float(hist[-1].mean()), "holdout_nmse_mean": float(holdout_nmse.mean()),
+        "n_query": n_query, "max_epochs": epochs, "epochs_trained": int(len(vhist) * check_every),
+        "best_epoch_per_member": [int((r + 1) * check_every) for r in best_round], "weight_decay": weight_decay,
+        "solver_seconds_per_sample": t_sample, "train_seconds": t_train,
+        "best_validation_error": float(vhist.min(0).mean()), "holdout_nmse_mean": float(holdout_nmse.mean()),
         "holdout_nmse_max": float(holdout_nmse.max()), "fallback_fraction": st["fallback_fraction"],
         "predict_seconds": st["predict_seconds"], "fallback_solver_seconds": st["solver_seconds"],
     }
@@ -336,6 +400,7 @@ def main(argv=None):
     ap.add_argument("--n-query", type=int, default=16384)
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--weight-decay", type=float, default=0.01)
     ap.add_argument("--chunk", type=int, default=2048)
     args = ap.parse_args(argv)
     os.makedirs(args.data, exist_ok=True)
@@ -371,7 +436,8 @@ def main(argv=None):
         data = {"train": tr["train"], "holdout": tr["holdout"], "ref": Yref,
                 "solver_seconds_per_sample": seconds / count}
         res, _ = uq_experiment(n_train=tr["train"].shape[0], n_holdout=tr["holdout"].shape[0], n_ref=Yref.shape[0],
-                               n_query=args.n_query, epochs=args.epochs, lr=args.lr, data=data, log=log, **kw)
+                               n_query=args.n_query, epochs=args.epochs, lr=args.lr, data=data, log=log,
+                               weight_decay=args.weight_decay, **kw)
         print(json.dumps(res), flush=True)
 
 

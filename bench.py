@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 SHAPE_NAMES = {1: ["256", "64"], 2: ["32x16", "16x15", "64x4", "32x15", "24x20"],
                3: ["32x8 columns x 32 planes", "16x15 columns x 32 planes", "32x7 columns x 32 planes",
                    "16x15 columns x 32 planes, 2 barriers", "16x15 columns x 16 planes, 2 barriers",
-                   "32x7 columns x 32 planes, 2 barriers"]}
+                   "32x7 columns x 32 planes, 2 barriers", "16x15 columns x 32 planes, 2 barriers, shared reciprocals"]}
 
 WORKLOADS = {
     # the bench line
@@ -166,12 +166,30 @@ def run_uq_to_end(args, br, cells, total):
     br.check_constraints()
     steps = [[br.progress(g, s).steps for s in range(m)] for g, (n, m) in enumerate(UQ_GROUPS)]
     updates = sum(n[0] * n[1] * sum(st) for (n, m), st in zip(UQ_GROUPS, steps))
+    group_updates = [n[0] * n[1] * sum(st) for (n, m), st in zip(UQ_GROUPS, steps)]
     stats = {}
+    eps = args.uq_target_error
     for g, (n, m) in enumerate(UQ_GROUPS):
         mean, var, cnt = br.statistics(g, 0, primitive=True)
+        # Monte Carlo error of the statistics of rho (L1 over the domain,
+        # relative): standard error of the mean sqrt(var / M), and of the
+        # variance sqrt(2 / (M - 1)) var (normal approximation)
+        l1 = lambda a: float(np.abs(a[0]).mean())
+        err_mean = l1(np.sqrt(var / cnt)) / l1(mean)
+        err_var = l1(np.sqrt(2.0 / (cnt - 1)) * var) / l1(var)
+        # samples are independent and cost the same, so the error falls as
+        # 1/sqrt(M) at a wall time proportional to M: the samples and wall
+        # time this group needs for relative errors <= eps (its share of the
+        # run's wall time by cell updates)
+        need = int(np.ceil(cnt * max(err_mean, err_var) ** 2 / eps ** 2))
+        share = wall * group_updates[g] / updates
         stats["%dx%d" % n[:2]] = {"samples": int(cnt), "steps_min_max": [min(steps[g]), max(steps[g])],
                                   "domain_mean_of_mean_rho_ux_uy_p": [float(v) for v in mean.mean(axis=(1, 2, 3))],
-                                  "domain_mean_of_variance_rho_ux_uy_p": [float(v) for v in var.mean(axis=(1, 2, 3))]}
+                                  "domain_mean_of_variance_rho_ux_uy_p": [float(v) for v in var.mean(axis=(1, 2, 3))],
+                                  "rho_mean_rel_std_error": err_mean, "rho_variance_rel_std_error": err_var,
+                                  "wall_share_s": share,
+                                  "samples_for_target_error": need,
+                                  "wall_to_target_error_s": share * need / cnt}
     return {
         "metric": "UQ wall time (all samples to T = 2, on-device mean/variance)",
         "value": wall,
@@ -183,6 +201,11 @@ def run_uq_to_end(args, br, cells, total):
         "config": {"workload": UQ_DESC, "cells": cells, "samples": total, "parallelism": "single GPU"},
         "cell_updates": updates,
         "cell_updates_per_s": updates / wall,
+        "target_rel_error": eps,
+        "wall_to_target_error_s": sum(v["wall_to_target_error_s"] for v in stats.values()),
+        "wall_to_target_error_note": ("per resolution: measured wall time share x (samples needed for relative "
+                                      "MC errors of mean and variance of rho <= target) / samples run; "
+                                      "samples are independent, so the cost is linear in their number"),
         "statistics": stats,
     }
 
@@ -285,7 +308,10 @@ def main():
                     help="fix the stage kernel's tile shape (index into the autotuner's list) instead of "
                          "autotuning, e.g. to profile one kernel instance")
     ap.add_argument("--uq-to-end", action="store_true",
-                    help="with --workload uq: run every sample to T = 2 and report the wall time")
+                    help="with --workload uq: run every sample to T = 2 and report the wall time, the Monte "
+                         "Carlo errors of the statistics and the wall time to --uq-target-error")
+    ap.add_argument("--uq-target-error", type=float, default=0.01,
+                    help="relative Monte Carlo error of the mean and variance of rho (L1 over the domain)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (torch copy kernel), to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters against a known byte count")

@@ -450,10 +450,15 @@ avh_status avh_mlp_set_params(avh_mlp* mlp, int member, const float* params);
  * X[rows][n_in], Y[rows][n_out], grads[member][num_params], loss[member] */
 avh_status avh_mlp_gradients(avh_mlp* mlp, const float* X, const float* Y, int64_t rows, float* grads,
                              float* loss);
+/* decoupled weight decay (AdamW) of the weights in every later Adam step:
+ * p -= lr * weight_decay * p beside the Adam update; biases are not
+ * decayed; 0 (the default) is plain Adam */
+avh_status avh_mlp_set_weight_decay(avh_mlp* mlp, float weight_decay);
 /* one Adam step of every member from the last gradients */
 avh_status avh_mlp_adam_step(avh_mlp* mlp, float lr);
 /* minibatch Adam over `epochs` passes, cosine learning-rate decay from lr to
- * lr * final_lr_fraction; epoch_loss[epoch][member] (may be NULL) */
+ * lr * final_lr_fraction over this call; the shuffles continue the epoch
+ * count of earlier calls; epoch_loss[epoch][member] (may be NULL) */
 avh_status avh_mlp_train(avh_mlp* mlp, const float* X, const float* Y, int64_t rows, int epochs, float lr,
                          float final_lr_fraction, float* epoch_loss);
 /* out[member][row][n_out] in data units */

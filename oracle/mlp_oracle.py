@@ -63,6 +63,11 @@ class MlpOracle:
     def num_params(self):
         return sum(o * i + o for o, i in self.shapes)
 
+    def weight_mask(self):
+        """1 for the weights, 0 for the biases, in the flat parameter order
+        (per layer W row-major, then b)."""
+        return np.concatenate([np.concatenate([np.ones(o * i), np.zeros(o)]) for o, i in self.shapes])
+
     def set_flat(self, flat):
         flat = np.asarray(flat, dtype=np.float64)
         o = 0
@@ -136,9 +141,14 @@ class MlpOracle:
         return yhat * so + mo
 
 
-def adam(params, grads, m, v, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
-    """One bias-corrected Adam step (t counts from 1); returns new arrays."""
+def adam(params, grads, m, v, t, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0, decay_mask=None):
+    """One bias-corrected Adam step (t counts from 1); returns new arrays.
+    weight_decay: decoupled decay (AdamW, Loshchilov and Hutter 2019),
+    p -= lr * weight_decay * p on the entries where decay_mask is 1."""
     m = beta1 * m + (1 - beta1) * grads
     v = beta2 * v + (1 - beta2) * grads * grads
     p = params - lr * (m / (1 - beta1 ** t)) / (np.sqrt(v / (1 - beta2 ** t)) + eps)
+    if weight_decay:
+        mask = np.ones_like(params) if decay_mask is None else decay_mask
+        p = p - lr * weight_decay * params * mask
     return p, m, v

@@ -88,6 +88,26 @@ def test_adam_steps_match_oracle():
             p[m] = got
 
 
+def test_adamw_steps_match_oracle():
+    """Decoupled weight decay on the weights (not the biases)."""
+    from oracle import mlp_oracle as mo
+    net, oracles, X, Y, norm = _setup(7)
+    net.set_weight_decay(0.05)
+    mask = oracles[0].weight_mask()
+    p = [net.get_params(m).astype(np.float64) for m in range(MEMBERS)]
+    mst = [np.zeros_like(x) for x in p]
+    vst = [np.zeros_like(x) for x in p]
+    for t in (1, 2, 3):
+        _, g = net.gradients(X, Y)
+        net.adam_step(1e-3)
+        for m in range(MEMBERS):
+            p[m], mst[m], vst[m] = mo.adam(p[m], g[m].astype(np.float64), mst[m], vst[m], t, 1e-3,
+                                           weight_decay=0.05, decay_mask=mask)
+            got = net.get_params(m).astype(np.float64)
+            np.testing.assert_allclose(got, p[m], rtol=2e-5, atol=2e-7)
+            p[m] = got
+
+
 def test_predict_matches_oracle_forward():
     net, oracles, X, Y, norm = _setup(4)
     rng = np.random.default_rng(5)

@@ -78,3 +78,28 @@ def test_surrogate_gains_on_a_smooth_problem():
     z = np.abs(st["cv_mean"] - Yref.mean(0)) / se
     _say("cv mean z-scores", np.round(z, 2), "fallback fraction", st["fallback_fraction"])
     assert np.all(z < 5)
+
+
+def test_early_stopping_keeps_each_members_best_round():
+    """On a target that is pure noise the validation error can only rise as
+    the network fits the training noise: every member must end at the
+    parameters of its best round, whose validation error is the minimum of
+    the rounds, and the optimal-beta control variate is never worse than
+    plain Monte Carlo (efficiency >= 1)."""
+    from alsvinn_amd.surrogate import EnsembleMLP, control_variate_efficiency
+    rng = np.random.default_rng(8)
+    X = rng.random((1024, 6)).astype(np.float32)
+    Y = (np.sin(3 * X[:, :1]) + rng.normal(size=(1024, 3))).astype(np.float32)
+    Xv = rng.random((256, 6)).astype(np.float32)
+    Yv = (np.sin(3 * Xv[:, :1]) + rng.normal(size=(256, 3))).astype(np.float32)
+    net = EnsembleMLP(6, 3, width=64, hidden=3, members=3, batch=64, seed=5)
+    net.fit_normalization(X, Y)
+    net.set_weight_decay(0.01)
+    hist, best = net.train_early_stopping(X, Y, Xv, Yv, max_epochs=120, check_every=10, lr=3e-3, patience=3)
+    _say("validation error per round", np.round(hist.mean(1), 4), "best rounds", best)
+    pm = net.predict_members(Xv).astype(np.float64)
+    sd = Yv.astype(np.float64).std(0)
+    final = np.mean(((pm - Yv[None]) / sd) ** 2, axis=(1, 2))
+    np.testing.assert_allclose(final, hist.min(0), rtol=1e-5)
+    eff = control_variate_efficiency(net, Xv, Yv)
+    assert np.all(eff >= 1.0 - 1e-12)

@@ -20,8 +20,8 @@ CASES = [
     ("euler2", "weno2", "hll3", "rungekutta3", "periodic", (70, 37, 1), 0.4, 5),
     ("euler2", "eno2", "hll", "forwardeuler", "neumann", (45, 29, 1), 0.2, 5),
     ("burgers", "none", "godunov", "rungekutta3", "periodic", (33, 66, 1), 0.4, 5),
-    ("euler3", "weno2", "hll3", "rungekutta3", "periodic", (35, 19, 40), 0.3, 6),
-    ("euler3", "eno3", "hll", "rungekutta2", "neumann", (17, 33, 36), 0.3, 6),
+    ("euler3", "weno2", "hll3", "rungekutta3", "periodic", (35, 19, 40), 0.3, 7),
+    ("euler3", "eno3", "hll", "rungekutta2", "neumann", (17, 33, 36), 0.3, 7),
 ]
 
 

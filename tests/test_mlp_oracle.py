@@ -71,6 +71,30 @@ def test_forward_and_gradients_match_torch_autograd():
     np.testing.assert_allclose(g, ref.numpy(), rtol=1e-10, atol=1e-14)
 
 
+def test_adamw_matches_torch():
+    """Decoupled weight decay on the weights only (biases in a group of
+    their own without decay), against torch.optim.AdamW."""
+    net, X, Y, norm = _net(5)
+    rng = np.random.default_rng(6)
+    p = net.flat().copy()
+    mask = net.weight_mask()
+    assert mask.sum() == sum(o * i for o, i in net.shapes) and mask.size == net.num_params()
+    pw = torch.tensor(p[mask == 1].copy(), requires_grad=True)
+    pb = torch.tensor(p[mask == 0].copy(), requires_grad=True)
+    opt = torch.optim.AdamW([{"params": [pw], "weight_decay": 0.05}, {"params": [pb], "weight_decay": 0.0}],
+                            lr=3e-3, betas=(0.9, 0.999), eps=1e-8)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t in range(1, 8):
+        g = rng.normal(size=p.size)
+        pw.grad = torch.tensor(g[mask == 1])
+        pb.grad = torch.tensor(g[mask == 0])
+        opt.step()
+        p, m, v = mo.adam(p, g, m, v, t, 3e-3, weight_decay=0.05, decay_mask=mask)
+        np.testing.assert_allclose(p[mask == 1], pw.detach().numpy(), rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(p[mask == 0], pb.detach().numpy(), rtol=1e-13, atol=1e-15)
+
+
 def test_adam_matches_torch():
     rng = np.random.default_rng(4)
     p = rng.normal(size=50)
