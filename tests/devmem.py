@@ -3,10 +3,10 @@ runtime's C API (hipMalloc / hipMemcpy / hipFree), and reference Volumes
 (one device array per conserved variable, ghost cells included) built from
 numpy arrays of shape (nvars, nz_total, ny_total, nx_total)."""
 import ctypes
-import os
 
 import numpy as np
 
+import alsvinn_amd
 from alsvinn_amd import _abi
 
 _hip = None
@@ -14,10 +14,14 @@ H2D, D2H = 1, 2
 
 
 def hip():
+    """The HIP runtime the hip platform library runs on.  Loaded by soname
+    after the library, so that it is the runtime object already in the
+    process (torch, when imported first, brings its own build of it, and a
+    second runtime from another path would not match torch's HSA runtime)."""
     global _hip
     if _hip is None:
-        rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-        L = ctypes.CDLL(os.path.join(rocm, "lib", "libamdhip64.so"))
+        alsvinn_amd.load()
+        L = ctypes.CDLL("libamdhip64.so.7")
         L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         L.hipFree.argtypes = [ctypes.c_void_p]
         L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
