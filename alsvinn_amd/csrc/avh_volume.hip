@@ -133,6 +133,16 @@ __global__ __launch_bounds__(256) void volume_reconstruct_kernel(ReconVolParams 
         const long long sx = 1, sy = R.tot[0], sz = (long long)R.tot[0] * R.tot[1];
         const long long c = (z + R.ghost[2]) * sz + (y + R.ghost[1]) * sy + (x + R.ghost[0]);
         const long long step = R.dir == 0 ? sx : (R.dir == 1 ? sy : sz);
+        if (RECON == RECON_NONE) {
+            // a copy of the cell itself (the range is the whole volume)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double q = R.in[v][c];
+                R.left[v][c] = q;
+                R.right[v][c] = q;
+            }
+            continue;
+        }
         double w[WZ][NV];
 #pragma unroll
         for (int k = 0; k < WZ; ++k)
@@ -378,12 +388,19 @@ avh_status avh_reconstruction(avh_ctx* ctx, const avh_config* cfg, const avh_vol
     for (int d = 0; d < 3; ++d) {
         if (d >= dim && (start[d] != 0 || end[d] != 0)) return fail(AVH_EINVAL, "start/end on an inactive axis");
         if (d < dim && (start[d] < 0 || end[d] > 0)) return fail(AVH_EINVAL, "start must be >= 0 and end <= 0");
-        const int widen = d == direction ? 1 : 0;
-        R.org[d] = start[d] - widen;
-        R.ext[d] = in->n[d] + end[d] + widen - R.org[d];
-        if (R.ext[d] < 0) R.ext[d] = 0;
         R.tot[d] = in->n[d] + 2 * in->ghosts[d];
         R.ghost[d] = in->ghosts[d];
+        if (cfg->reconstruction == AVH_RECON_NONE) {
+            // NoReconstruction copies every cell of the volume, ghost cells
+            // included, whatever the range (NoReconstruction.cpp:46-60)
+            R.org[d] = -in->ghosts[d];
+            R.ext[d] = R.tot[d];
+        } else {
+            const int widen = d == direction ? 1 : 0;
+            R.org[d] = start[d] - widen;
+            R.ext[d] = in->n[d] + end[d] + widen - R.org[d];
+            if (R.ext[d] < 0) R.ext[d] = 0;
+        }
         total *= R.ext[d];
     }
     for (int v = 0; v < nv; ++v) {

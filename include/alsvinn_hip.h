@@ -255,7 +255,9 @@ avh_status avh_numerical_flux(avh_ctx* ctx, const avh_config* cfg, const avh_vol
  * upper face along `direction` of every cell with interior coordinates in
  * [start[d] - (d == direction), n[d] + end[d] + (d == direction)).  ENO
  * chooses its stencils on `indicator_variable`; WENO2 weights on the
- * equation's weight variable (rho / u), as the reference. */
+ * equation's weight variable (rho / u), as the reference.  Reconstruction
+ * "none" copies every cell of the volume, ghost cells included, whatever
+ * start/end say (alsfvm/src/reconstruction/NoReconstruction.cpp:46-60). */
 avh_status avh_reconstruction(avh_ctx* ctx, const avh_config* cfg, const avh_volume* in, int direction,
                               int indicator_variable, avh_volume* left, avh_volume* right,
                               const int start[3], const int end[3], void* stream);
