@@ -351,11 +351,12 @@ def main():
     sim.set_interior(initial_state(wl))
     dim = 3 if wl["n"][2] > 1 else (2 if wl["n"][1] > 1 else 1)
     if args.tile_shape >= 0:
-        sim.set_tile_shape(args.tile_shape)
-        tune = {"chosen": SHAPE_NAMES[dim][args.tile_shape], "fixed": True}
+        shape = args.tile_shape
+        sim.set_tile_shape(shape)
+        tune = {"chosen": SHAPE_NAMES[dim][shape], "fixed": True}
     else:
-        best, tune_ms = sim.autotune()
-        tune = {"chosen": SHAPE_NAMES[dim][best], "ms_per_shape": dict(zip(SHAPE_NAMES[dim], tune_ms))}
+        shape, tune_ms = sim.autotune()
+        tune = {"chosen": SHAPE_NAMES[dim][shape], "ms_per_shape": dict(zip(SHAPE_NAMES[dim], tune_ms))}
     sim.prepare_graph()  # capture + instantiate now: the timed steps only replay
 
     sim.enqueue_steps(args.warmup)
@@ -391,6 +392,9 @@ def main():
     valu = None
     pname = "stage_kernel_%s_traffic.json" % WORKLOADS[args.workload]["profile"]
     prof, stale = load_profile(pname)
+    if prof and prof.get("tile_shape", shape) != shape:
+        prof, stale = None, "%s was measured on tile shape %s, this run used %s" % (
+            pname, SHAPE_NAMES[dim][prof["tile_shape"]], SHAPE_NAMES[dim][shape])
     if prof:
         traffic = prof.get("bytes_per_launch")
         insts = prof.get("valu_insts_per_launch")
