@@ -3,7 +3,7 @@
 // every member in the same launches (blockIdx.z = member).
 //
 // Arithmetic: weights, activations and backward signals are stored in bf16;
-// every product runs on the matrix cores (v_mfma_f32_16x16x16_bf16) with
+// every product runs on the matrix cores (v_mfma_f32_16x16x32_bf16, with one 16x16x16 step for a K tail of 16) with
 // fp32 accumulation; biases, pre-activations, outputs, gradients and the
 // Adam state are fp32.  Feature dimensions are padded to multiples of 16
 // with zeros (the padded weights get zero gradients and stay zero).
@@ -35,6 +35,7 @@ namespace {
 constexpr int MAX_LAYERS = 8;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -83,12 +84,22 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g) {
     const int tm = blockIdx.y * 2 + (w >> 1), tn = blockIdx.x * 2 + (w & 1);
     if (tm * 16 >= g.M || tn * 16 >= g.N) return;
     const int mem = blockIdx.z;
-    const uint16_t* A = g.A + mem * g.a_ms + (long long)(tm * 16 + (lane & 15)) * g.lda + 4 * (lane >> 4);
-    const uint16_t* B = g.B + mem * g.b_ms + (long long)(tn * 16 + (lane & 15)) * g.ldb + 4 * (lane >> 4);
+    const uint16_t* A = g.A + mem * g.a_ms + (long long)(tm * 16 + (lane & 15)) * g.lda;
+    const uint16_t* B = g.B + mem * g.b_ms + (long long)(tn * 16 + (lane & 15)) * g.ldb;
+    const int kq = lane >> 4;  // the lane's quarter of each K step
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4*>(A + k0);
-        const bf16x4 b = *reinterpret_cast<const bf16x4*>(B + k0);
+    // K steps of 32 on gfx950's v_mfma_f32_16x16x32_bf16 (8 consecutive k
+    // per lane, 16-byte loads), then a step of 16 for K = 16 mod 32 (the
+    // padded widths 48 and 80)
+    int k0 = 0;
+    for (; k0 + 32 <= g.K; k0 += 32) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(A + k0 + 8 * kq);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(B + k0 + 8 * kq);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    if (k0 < g.K) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(A + k0 + 4 * kq);
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(B + k0 + 4 * kq);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
     const int n = tn * 16 + (lane & 15);

@@ -56,7 +56,11 @@ __device__ __forceinline__ double div_shared(double n, double d, double r) {
 // n is zero or lo <= |n| <= hi (false for inf and NaN)
 __device__ __forceinline__ bool zero_or_within(double n, double lo, double hi) {
     const double a = fabs(n);
-    return a <= hi && (a >= lo || n == 0.0);
+    return (a <= hi) & ((a >= lo) | (n == 0.0));
+}
+// lo <= n <= hi (false for zero, negative values, inf and NaN)
+__device__ __forceinline__ bool positive_within(double n, double lo, double hi) {
+    return (n >= lo) & (n <= hi);
 }
 
 // Arithmetic of an interface's reconstruction and flux.
@@ -68,16 +72,23 @@ __device__ __forceinline__ bool zero_or_within(double n, double lo, double hi) {
 //              the caller recomputes the interface with PlainOps.  The
 //              checks feed a flag instead of branching so that the interface
 //              stays one straight-line block for the scheduler.
+// The checks combine with non-short-circuit '&' so that they stay lane
+// masks and never turn into branches.
 struct PlainOps {
     static constexpr bool SHARED = false;
     __device__ __forceinline__ static double sqrt_(double x, bool&) { return sqrt(x); }
+    // sqrt of an argument already known to lie in [2^-767, DBL_MAX] under
+    // SharedOps (a density that passed make_all_ops' check, or a quotient
+    // of checked operands)
+    __device__ __forceinline__ static double sqrt_in_range(double x) { return sqrt(x); }
 };
 struct SharedOps {
     static constexpr bool SHARED = true;
     __device__ __forceinline__ static double sqrt_(double x, bool& ok) {
-        ok = ok && x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023;
+        ok = ok & (x >= 0x1p-767) & (x <= 0x1.fffffffffffffp+1023);
         return sqrt_core(x);
     }
+    __device__ __forceinline__ static double sqrt_in_range(double x) { return sqrt_core(x); }
 };
 template <bool SHARED>
 struct OpsFor {
@@ -157,9 +168,9 @@ struct EulerEq {
 #pragma unroll
             for (int c = 0; c < NSD_; ++c) a.m[c] = q[1 + c];
             a.E = q[NSD_ + 1];
-            ok = ok && a.rho >= 0x1p-100 && a.rho <= 0x1p100;
+            ok = ok & positive_within(a.rho, 0x1p-100, 0x1p100);
 #pragma unroll
-            for (int c = 0; c < NSD_; ++c) ok = ok && zero_or_within(a.m[c], 0x1p-390, 0x1p290);
+            for (int c = 0; c < NSD_; ++c) ok = ok & zero_or_within(a.m[c], 0x1p-390, 0x1p290);
             const double r = div_recip(a.rho);
             const double ie = a.E - div_shared(0.5 * dot(a.m), a.rho, r);
 #pragma unroll
@@ -176,8 +187,20 @@ struct EulerEq {
         if constexpr (!OPS::SHARED) {
             return n / a.rho;
         } else {
-            ok = ok && zero_or_within(n, 0x1p-800, 0x1p600);
+            ok = ok & zero_or_within(n, 0x1p-800, 0x1p600);
             return div_shared(n, a.rho, a.rinv);
+        }
+    }
+    // sqrt(n / rho) for a state from make_all_ops<OPS>; SharedOps checks that
+    // n lies in [2^-600, 2^600], so that the quotient lies in [2^-700, 2^700],
+    // inside sqrt_core's range (the square root of a wave speed: n = gamma*p)
+    template <class OPS>
+    __device__ __forceinline__ static double sqrt_div_rho(double n, const All& a, bool& ok) {
+        if constexpr (!OPS::SHARED) {
+            return sqrt(n / a.rho);
+        } else {
+            ok = ok & positive_within(n, 0x1p-600, 0x1p600);
+            return sqrt_core(div_shared(n, a.rho, a.rinv));
         }
     }
     // (n1, n2) / (wl + wr) with wl, wr the square roots of the two sides'
@@ -191,7 +214,7 @@ struct EulerEq {
             q1 = n1 / (wl + wr);
             q2 = n2 / (wl + wr);
         } else {
-            ok = ok && zero_or_within(n1, 0x1p-800, 0x1p600) && zero_or_within(n2, 0x1p-800, 0x1p600);
+            ok = ok & zero_or_within(n1, 0x1p-800, 0x1p600) & zero_or_within(n2, 0x1p-800, 0x1p600);
             const double den = wl + wr;
             const double r = div_recip(den);
             q1 = div_shared(n1, den, r);
@@ -263,14 +286,14 @@ struct HLLFluxT {
                                                       double g, double* F, bool& ok) {
         constexpr int NSD = EQ::NSD;
         constexpr int NV = EQ::NV;
-        const double wl = OPS::sqrt_(L.rho, ok);
-        const double wr = OPS::sqrt_(R.rho, ok);
+        const double wl = OPS::sqrt_in_range(L.rho);
+        const double wr = OPS::sqrt_in_range(R.rho);
         const double rho = (L.rho + R.rho) / 2;
         double ud, p;
         EQ::template div_roe<OPS>(L.u[DIR] * wl + R.u[DIR] * wr, L.p * wl + R.p * wr, wl, wr, ud, p, ok);
         const double cs = OPS::sqrt_(g * p / rho, ok);
-        const double cfl = OPS::sqrt_(EQ::template div_rho<OPS>(g * L.p, L, ok), ok);
-        const double cfr = OPS::sqrt_(EQ::template div_rho<OPS>(g * R.p, R, ok), ok);
+        const double cfl = EQ::template sqrt_div_rho<OPS>(g * L.p, L, ok);
+        const double cfr = EQ::template sqrt_div_rho<OPS>(g * R.p, R, ok);
         const double sl = fmin(L.u[DIR] - cfl, ud - cs);
         const double sr = fmax(R.u[DIR] + cfr, ud + cs);
         double fl[NV], fr[NV], ul[NV], ur[NV];
@@ -319,13 +342,13 @@ struct HLL3FluxT {
         constexpr int NSD = EQ::NSD;
         constexpr int NV = EQ::NV;
         // computeHLLSpeeds
-        const double wl = OPS::sqrt_(L.rho, ok);
-        const double wr = OPS::sqrt_(R.rho, ok);
+        const double wl = OPS::sqrt_in_range(L.rho);
+        const double wr = OPS::sqrt_in_range(R.rho);
         const double rho = (L.rho + R.rho) / 2;
         double ud, p;
         EQ::template div_roe<OPS>(L.u[DIR] * wl + R.u[DIR] * wr, wl * L.p + wr * R.p, wl, wr, ud, p, ok);
-        const double cfl = OPS::sqrt_(EQ::template div_rho<OPS>(g * L.p, L, ok), ok);
-        const double cfr = OPS::sqrt_(EQ::template div_rho<OPS>(g * R.p, R, ok), ok);
+        const double cfl = EQ::template sqrt_div_rho<OPS>(g * L.p, L, ok);
+        const double cfr = EQ::template sqrt_div_rho<OPS>(g * R.p, R, ok);
         const double correct = 0.5 * fmax(0.0, L.u[DIR] - R.u[DIR]);
         const double cs = OPS::sqrt_(g * p / rho, ok);
         const double sl = fmin(L.u[DIR] + correct - cfl, ud - cs);
@@ -423,7 +446,7 @@ struct ReconTraits<RECON_ENO3> {
 template <class OPS>
 __device__ __forceinline__ void weno2_weights(double a0, double a1, double& w0, double& w1, bool& ok) {
     if constexpr (OPS::SHARED) {
-        ok = ok && a0 >= 0x1p-400 && a1 >= 0x1p-400;
+        ok = ok & (a0 >= 0x1p-400) & (a1 >= 0x1p-400);
         const double den = a0 + a1;
         const double r = div_recip(den);
         w0 = div_shared(a0, den, r);

@@ -182,8 +182,12 @@ avh_status avh_prepare_graph(avh_solver* solver);
 /* Tile shape of the fused stage kernel: -1 (default) lets the autotuner
  * pick on first use (cached per equation, reconstruction, flux and grid for
  * the process), 0..nshapes-1 fixes one (1D: 256 or 64 cells; 2D: 32x16,
- * 32x8, 64x4, 32x15 or 24x20; 3D: 32x8, 16x15 or 32x7 columns marching over
- * 32 planes).  Results are identical for every shape.
+ * 16x15, 64x4, 32x15 or 24x20; 3D columns marching over z-planes: 32x8,
+ * 16x15 and 32x7 columns over 32 planes with one plane buffer, then 16x15
+ * over 32 and over 16 planes and 32x7 over 32 planes with a double-buffered
+ * plane and two barriers per plane, then 16x15 over 32 planes, two
+ * barriers, fluxes with shared reciprocals).  Results are identical for
+ * every shape.
  * No reference counterpart: the reference's CUDA kernels use fixed block
  * sizes (alsfvm/src/numflux/NumericalFluxCUDA.cu:330-414). */
 avh_status avh_set_tile_shape(avh_solver* solver, int shape);

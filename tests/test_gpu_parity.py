@@ -269,3 +269,101 @@ def test_unfused_reconstruction_bitwise(recon, eq, n, bc):
         lo, ro = pyoracle.reconstruct(cfg, U, d)
         np.testing.assert_array_equal(left, h.interior(lo))
         np.testing.assert_array_equal(right, h.interior(ro))
+
+
+def _upper_extreme_state(equation, n, kind, seed=12):
+    """Smooth states that cross an upper bound of the shortcuts along x, so
+    that interfaces on both sides of it share wavefronts.  Densities are
+    about 1e29 (inside the shortcuts' density range) so that wave speeds stay
+    small enough for every flux product to remain finite.
+      "momentum": the first velocity rises and falls (periodically) as
+                  10^(57.5..59), the momentum through 2^290, the energy a
+                  hundred times the kinetic energy;
+      "pressure": velocities O(1), the pressure as 10^(179..182), so
+                  gamma*p and the Roe pressure numerator pass 2^600 (HLL's
+                  products overflow there; HLL3's do not)."""
+    q = smooth_state(equation, n, seed=seed)
+    nsd = q.shape[0] - 2
+    nx = q.shape[-1]
+    x = np.sin(np.pi * (np.arange(nx) + 0.5) / nx) ** 2
+    u = [q[1 + c] / q[0] for c in range(nsd)]
+    p = 0.4 * (q[-1] - 0.5 * sum(q[1 + c] ** 2 for c in range(nsd)) / q[0])
+    q[0] = q[0] * 1e29
+    if kind == "momentum":
+        u[0] = (1.5 + np.tanh(u[0])) * 10.0 ** (57.5 + 1.5 * x)
+    else:
+        p = p * 10.0 ** (179.0 + 3.0 * x)
+    for c in range(nsd):
+        q[1 + c] = q[0] * u[c]
+    kin = 0.5 * sum(q[1 + c] ** 2 for c in range(nsd)) / q[0]
+    q[-1] = 100.0 * kin + p / 0.4 if kind == "momentum" else kin + p / 0.4
+    return q
+
+
+UPPER_CASES = [(eq, n, recon, flux) for eq, n in (("euler1", (70,)), ("euler2", (40, 20)))
+               for recon in ("none", "weno2", "eno2", "eno3") for flux in ("hll", "hll3")] + \
+              [("euler3", (20, 18, 24), recon, "hll3") for recon in ("weno2", "eno3")] + \
+              [("euler3", (20, 18, 24), "weno2", "hll")]
+
+
+@pytest.mark.parametrize("equation,n,recon,flux", UPPER_CASES)
+def test_shared_reciprocal_upper_guards_bitwise(equation, n, recon, flux):
+    """Operands above the shortcuts' upper bounds (momenta > 2^290; gamma*p
+    and the Roe pressure numerator > 2^600) next to operands below them,
+    with every reconstruction: the net flux and wave speeds keep every bit
+    of the oracle's, and so does a whole RK3 step (state and dt).  In 3D
+    every march-kernel tile shape runs, the shapes with shared reciprocals
+    included."""
+    from oracle import pyoracle
+    shapes = range(7) if len(n) == 3 else [-1]  # the 3D march-kernel tile shapes
+    for kind in ("momentum", "pressure") if flux == "hll3" else ("momentum",):
+        _say("upper guards", equation, n, recon, flux, kind)
+        q0 = _upper_extreme_state(equation, n, kind)
+        for shape in shapes:
+            o, h = make_pair(pyoracle, equation, recon, flux, "rungekutta3", "periodic", n, 0.4)
+            o.set_interior(q0)
+            h.set_interior(q0)
+            if shape >= 0:
+                h.set_tile_shape(shape)
+            U = o.get_state()
+            out_o, ws_o = pyoracle.compute_flux(o.cfg, U, np.zeros_like(U))
+            out_h, ws_h = h.compute_flux()
+            ref = np.ascontiguousarray(o.interior(out_o))
+            assert np.isfinite(ref).all()
+            assert np.array_equal(np.asarray(ws_h).view(np.uint64), np.asarray(ws_o).view(np.uint64)), shape
+            assert np.array_equal(np.ascontiguousarray(out_h).view(np.uint64), ref.view(np.uint64)), shape
+            dt_o = o.step()
+            dt_h = h.perform_step()
+            assert dt_h == dt_o
+            assert np.array_equal(h.get_interior().view(np.uint64), o.get_interior().view(np.uint64)), shape
+
+
+@pytest.mark.parametrize("equation,n", [("euler2", (40, 20)), ("euler3", (20, 18, 24))])
+def test_negative_pressure_cells_flux_and_step(equation, n):
+    """Cells of negative pressure (square roots of negative arguments, NaN
+    wave speeds): the flux equals the oracle's wherever the oracle's is a
+    number (NaN where it is NaN), and the step raises the constraint
+    violation as the reference does."""
+    from oracle import pyoracle
+    _say("negative pressure", equation, n)
+    q0 = smooth_state(equation, n, seed=13)
+    flat = q0.reshape(q0.shape[0], -1)
+    neg = np.random.default_rng(13).permutation(flat.shape[1])[:max(1, flat.shape[1] // 50)]
+    kin = 0.5 * sum(flat[1 + c, neg] ** 2 for c in range(flat.shape[0] - 2)) / flat[0, neg]
+    flat[-1, neg] = kin - 0.5  # p < 0
+    o, h = make_pair(pyoracle, equation, "weno2", "hll3", "rungekutta3", "periodic", n, 0.4)
+    o.set_interior(q0)
+    h.set_interior(q0)
+    U = o.get_state()
+    out_o, ws_o = pyoracle.compute_flux(o.cfg, U, np.zeros_like(U))
+    out_h, ws_h = h.compute_flux()
+    ref = np.ascontiguousarray(o.interior(out_o))
+    assert np.isnan(ref).any()
+    np.testing.assert_array_equal(np.isnan(out_h), np.isnan(ref))
+    fin = ~np.isnan(ref)
+    assert np.array_equal(out_h[fin].view(np.uint64), ref[fin].view(np.uint64))
+    np.testing.assert_array_equal(ws_h, ws_o)
+    with pytest.raises(pyoracle.OracleError):
+        o.step()
+    with pytest.raises(alsvinn_amd.ConstraintError):
+        h.perform_step()
