@@ -72,6 +72,18 @@ def load_profile(name):
     return prof, None
 
 
+def valu_bound(prof, avg_ms, source):
+    """The stage kernel's second bound, FP64 VALU issue: the profile's VALU
+    wave-instructions per launch over the launch time measured here, against
+    1024 SIMDs x one wave64 instruction per 4 cycles at the rated 2.4 GHz."""
+    insts = prof.get("valu_insts_per_launch") if prof else None
+    if not insts:
+        return None
+    peak = 1024 * 2.4e9 / 4
+    return {"insts_per_launch": insts, "achieved_per_s": insts / (avg_ms * 1e-3), "peak_per_s": peak,
+            "frac": insts / (avg_ms * 1e-3) / peak, "source": source}
+
+
 def host_cpu():
     model = None
     try:
@@ -226,11 +238,12 @@ def run_uq(args):
     if args.uq_to_end:
         return run_uq_to_end(args, br, cells, total)
     if args.tile_shape >= 0:
-        br.set_tile_shape(args.tile_shape)
-        tune = {"chosen": SHAPE_NAMES[2][args.tile_shape], "fixed": True}
+        shape = args.tile_shape
+        br.set_tile_shape(shape)
+        tune = {"chosen": SHAPE_NAMES[2][shape], "fixed": True}
     else:
-        best, tune_ms = br.autotune()
-        tune = {"chosen": SHAPE_NAMES[2][best], "ms_per_shape": dict(zip(SHAPE_NAMES[2], tune_ms))}
+        shape, tune_ms = br.autotune()
+        tune = {"chosen": SHAPE_NAMES[2][shape], "ms_per_shape": dict(zip(SHAPE_NAMES[2], tune_ms))}
     br.run(max_steps=args.warmup, chunk=max(1, args.warmup))
     print("bench: warmup done", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -245,8 +258,13 @@ def run_uq(args):
     avg_ms = sum(stage_ms) / 3
     avg_bytes = cells * sum(sb) / len(sb)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    prof, stale = load_profile("batch_stage_kernel_uq_traffic.json")
+    pname = "batch_stage_kernel_uq_traffic.json"
+    prof, stale = load_profile(pname)
+    if prof and prof.get("tile_shape", shape) != shape:
+        prof, stale = None, "%s was measured on tile shape %s, this run used %s" % (
+            pname, SHAPE_NAMES[2][prof["tile_shape"]], SHAPE_NAMES[2][shape])
     traffic = prof.get("bytes_per_launch") if prof else None
+    valu = valu_bound(prof, avg_ms, pname)
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
@@ -288,7 +306,9 @@ def run_uq(args):
             "stage_ms": stage_ms,
             "end_of_step_kernel_ms": launch_ms[3],
             "statistics_kernel_ms": launch_ms[4] if len(launch_ms) > 4 else None,
+            "kernel_source_digest": kernel_source_digest(),
             "tile_autotune": tune,
+            "valu": valu,
         },
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
@@ -397,12 +417,7 @@ def main():
             pname, SHAPE_NAMES[dim][prof["tile_shape"]], SHAPE_NAMES[dim][shape])
     if prof:
         traffic = prof.get("bytes_per_launch")
-        insts = prof.get("valu_insts_per_launch")
-        if insts:
-            # second bound: FP64 VALU issue, 1024 SIMDs x one wave64 instruction per 4 cycles at 2.4 GHz
-            peak = 1024 * 2.4e9 / 4
-            valu = {"insts_per_launch": insts, "achieved_per_s": insts / (avg_ms * 1e-3), "peak_per_s": peak,
-                    "frac": insts / (avg_ms * 1e-3) / peak, "source": pname}
+        valu = valu_bound(prof, avg_ms, pname)
 
     cpu = None
     if not args.no_cpu_baseline:
