@@ -485,9 +485,12 @@ static TuneKey tune_key(const avh_solver* s) {
                    s->cfg.n[2]);
 }
 
-// Times every tile shape with flux-only launches on the current state (they
-// write the L0 buffer, which stage 1 overwrites, and add to the wave-speed
-// maxima, which are restored) and keeps the fastest.
+// Times every tile shape on the current state and keeps the fastest: a
+// flux-only launch (writing the L0 buffer, which stage 1 overwrites), and
+// for RK3 also the second stage on its output (input formed on the fly from
+// L0 and U0, writing the U2 buffer), whose extra reads weigh differently on
+// different shapes.  The state is not written; the wave-speed maxima the
+// launches add to are restored.
 static avh_status autotune(avh_solver* s, int* best, double* ms, int* nshapes) {
     const int ns = num_shapes(s->dim);
     avh_status r = pull(s);
@@ -503,12 +506,23 @@ static avh_status autotune(avh_solver* s, int* best, double* ms, int* nshapes) {
     P.u0 = s->A;
     P.out = s->B;
     P.mode = MODE_FLUX_ONLY;
+    StageParams P2 = P;
+    P2.in = s->B;
+    P2.out = s->C;
+    P2.mode = MODE_RK3_2;
+    const bool rk3 = s->cfg.integrator == AVH_INT_RUNGEKUTTA3 && s->h.active;
     int b = 0;
     double tmin = 0;
     for (int k = 0; k < ns; ++k) {
         double t = 0;
         if (ns > 1) {
-            r = time_launches(s->stream, 5, [&]() { return s->stage(P, k, s->stream); }, &t);
+            r = time_launches(s->stream, 5,
+                              [&]() {
+                                  hipError_t e = s->stage(P, k, s->stream);
+                                  if (e == hipSuccess && rk3) e = s->stage(P2, k, s->stream);
+                                  return e;
+                              },
+                              &t);
             if (r) return r;
         }
         if (ms) ms[k] = t;

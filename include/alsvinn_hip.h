@@ -191,9 +191,10 @@ avh_status avh_prepare_graph(avh_solver* solver);
  * No reference counterpart: the reference's CUDA kernels use fixed block
  * sizes (alsfvm/src/numflux/NumericalFluxCUDA.cu:330-414). */
 avh_status avh_set_tile_shape(avh_solver* solver, int shape);
-/* Time every tile shape on the solver's current state now (flux-only
- * launches; the solution and step state are unchanged) and select the
- * fastest.  ms_per_shape (may be NULL) receives *nshapes <= 8 timings. */
+/* Time every tile shape on the solver's current state now (a flux-only
+ * launch, and for RK3 the second stage on its output, into the solver's
+ * scratch buffers; the solution and step state are unchanged) and select
+ * the fastest.  ms_per_shape (may be NULL) receives *nshapes <= 8 timings. */
 avh_status avh_autotune(avh_solver* solver, int* best_shape, double* ms_per_shape, int* nshapes);
 
 /* Run `steps` full time steps with a HIP event pair around every kernel
