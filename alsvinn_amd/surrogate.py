@@ -197,13 +197,35 @@ def kh_parameters(sample_start, nsamples):
     return generate_parameters([("a", 40, 0.0, 1.0)], sample_start, nsamples)["a"]
 
 
-def surrogate_statistics(model, X_query, solver=None, threshold=0.05, X_solved=None, Y_solved=None):
+def spread_calibration(model, X_solved, Y_solved, threshold=0.05):
+    """Whether the ensemble spread is a trustworthy error indicator, per
+    observable: among solver samples the network did not train on whose
+    spread passes the fallback test (spread <= threshold |mean| for every
+    observable), the fraction whose prediction is within threshold |mean|
+    of the solver's value.  An ensemble whose members agree with each other
+    but not with the solver (all of them near the mean of a map they did not
+    learn) passes the spread test and fails this one."""
+    g, spread = model.predict(X_solved)
+    y = np.asarray(Y_solved, dtype=np.float64)
+    accepted = ~np.any(spread > threshold * np.abs(g), axis=1)
+    if not accepted.any():
+        return np.zeros(g.shape[1]), 0
+    within = np.abs(g[accepted] - y[accepted]) <= threshold * np.abs(g[accepted])
+    return within.mean(0), int(accepted.sum())
+
+
+def surrogate_statistics(model, X_query, solver=None, threshold=0.05, X_solved=None, Y_solved=None,
+                         min_coverage=None):
     """Statistics of the observables over the query samples.
 
     fallback:  the ensemble mean where the ensemble spread is within
                `threshold` of |mean| for every observable, the solver
                (`solver(X) -> Y`) elsewhere; mean and population variance
-               over all query samples.
+               over all query samples.  With `min_coverage` (and solver
+               samples X_solved, Y_solved) the spread is trusted only if it
+               is calibrated: every observable's coverage
+               (spread_calibration) is at least min_coverage; otherwise
+               every query sample goes to the solver.
     control variate (when solver samples X_solved, Y_solved are given):
                E[f] = mean_S(f) - beta (mean_S(g) - mean_Q(g)) with the
                variance-minimising beta = cov_S(f, g) / var_S(g) per
@@ -219,6 +241,14 @@ def surrogate_statistics(model, X_query, solver=None, threshold=0.05, X_solved=N
     g, spread = model.predict(X_query)
     res["predict_seconds"] = time.perf_counter() - t0
     fallback = np.any(spread > threshold * np.abs(g), axis=1)
+    res["spread_fallback_fraction"] = float(fallback.mean())
+    if min_coverage is not None and X_solved is not None:
+        cov, n_acc = spread_calibration(model, X_solved, Y_solved, threshold)
+        res["spread_coverage"] = cov
+        res["spread_coverage_samples"] = n_acc
+        res["spread_trusted"] = bool(n_acc > 0 and np.all(cov >= min_coverage))
+        if not res["spread_trusted"]:
+            fallback = np.ones_like(fallback)
     res["fallback_fraction"] = float(fallback.mean())
     Y = g.copy()
     res["solver_seconds"] = 0.0
@@ -280,7 +310,8 @@ def plain_mc_error(pool, ref_mean, ref_var, nsamples, reps=16, seed=0):
 
 def uq_experiment(n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), number_of_h=63, n_train=4096, n_holdout=512,
                   n_ref=16384, n_query=16384, epochs=300, lr=1e-3, members=5, width=256, hidden=4, batch=256,
-                  threshold=0.05, chunk=2048, data=None, log=None, seed=0, weight_decay=0.01, check_every=10):
+                  threshold=0.05, chunk=2048, data=None, log=None, seed=0, weight_decay=0.01, check_every=10,
+                  min_coverage=0.9):
     """Surrogate-assisted UQ of the Kelvin-Helmholtz observables against plain
     Monte Carlo.  Sample ids: training [0, n_train), held out next, then the
     plain-solver reference, then the surrogate's query samples.  `data` may
@@ -333,7 +364,8 @@ def uq_experiment(n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), number_of_h=6
         log("fallback solver: %d samples in %.1f s" % (X.shape[0], time.perf_counter() - start))
         return out
 
-    st = surrogate_statistics(model, Xq, solver=solver, threshold=threshold, X_solved=Xho, Y_solved=Yho)
+    st = surrogate_statistics(model, Xq, solver=solver, threshold=threshold, X_solved=Xho, Y_solved=Yho,
+                              min_coverage=min_coverage)
     res = {
         "n": n, "end_time": end_time, "n_train": n_train, "n_holdout": n_holdout, "n_ref": n_ref,
         "n_query": n_query, "max_epochs": epochs, "epochs_trained": int(len(vhist) * check_every),
@@ -341,6 +373,9 @@ def uq_experiment(n=256, end_time=2.0, times=(0.5, 1.0, 1.5, 2.0), number_of_h=6
         "solver_seconds_per_sample": t_sample, "train_seconds": t_train,
         "best_validation_error": float(vhist.min(0).mean()), "holdout_nmse_mean": float(holdout_nmse.mean()),
         "holdout_nmse_max": float(holdout_nmse.max()), "fallback_fraction": st["fallback_fraction"],
+        "spread_fallback_fraction": st["spread_fallback_fraction"], "spread_trusted": st.get("spread_trusted"),
+        "spread_coverage_min": float(np.min(st["spread_coverage"])) if "spread_coverage" in st else None,
+        "spread_coverage_samples": st.get("spread_coverage_samples"),
         "predict_seconds": st["predict_seconds"], "fallback_solver_seconds": st["solver_seconds"],
     }
     # errors of each estimator against the plain-solver reference
@@ -402,6 +437,9 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--weight-decay", type=float, default=0.01)
     ap.add_argument("--chunk", type=int, default=2048)
+    ap.add_argument("--min-coverage", type=float, default=0.9,
+                    help="trust the ensemble spread only if, on held-out solver samples, at least this fraction "
+                         "of the accepted predictions is within the threshold of the solver's value")
     args = ap.parse_args(argv)
     os.makedirs(args.data, exist_ok=True)
     t0 = time.perf_counter()
@@ -437,7 +475,7 @@ def main(argv=None):
                 "solver_seconds_per_sample": seconds / count}
         res, _ = uq_experiment(n_train=tr["train"].shape[0], n_holdout=tr["holdout"].shape[0], n_ref=Yref.shape[0],
                                n_query=args.n_query, epochs=args.epochs, lr=args.lr, data=data, log=log,
-                               weight_decay=args.weight_decay, **kw)
+                               weight_decay=args.weight_decay, min_coverage=args.min_coverage, **kw)
         print(json.dumps(res), flush=True)
 
 
