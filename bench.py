@@ -33,11 +33,11 @@ SHAPE_NAMES = {1: ["256", "64"], 2: ["32x16", "16x15", "64x4", "32x15", "24x20"]
 
 WORKLOADS = {
     # the bench line
-    "kh2d": dict(equation="euler2", n=(1024, 1024, 1), nv=4, cfl=0.4, profile="kh2d_1024", mall_resident=True,
+    "kh2d": dict(equation="euler2", n=(1024, 1024, 1), nv=4, cfl=0.4, t_end=2.0, profile="kh2d_1024", mall_resident=True,
                  desc="2D compressible Euler Kelvin-Helmholtz 1024x1024, periodic, HLL3 + WENO2 + RK3-TVD, "
                       "CFL 0.40, gamma 1.4, FP64"),
     # 3D KH: working set beyond the 256 MB last-level cache (the %HBM case)
-    "kh3d": dict(equation="euler3", n=(256, 256, 256), nv=5, cfl=0.3, profile="kh3d_256", mall_resident=False,
+    "kh3d": dict(equation="euler3", n=(256, 256, 256), nv=5, cfl=0.3, t_end=1.0, profile="kh3d_256", mall_resident=False,
                  desc="3D compressible Euler Kelvin-Helmholtz 256^3, periodic, HLL3 + WENO2 + RK3-TVD, "
                       "CFL 0.30, gamma 1.4, FP64"),
 }
@@ -222,6 +222,35 @@ def run_uq_to_end(args, br, cells, total):
     }
 
 
+def run_to_end(sim, wl, tune):
+    """The whole run of a single-grid workload: from the initial data to its
+    end time (the last step lands on it, as with one writer save there), the
+    state resident on the device throughout."""
+    T = wl["t_end"]
+    sim.add_fixed_interval_writer(T)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    p = sim.run(T, chunk=512)
+    wall = time.perf_counter() - t0
+    if p.violated:
+        raise SystemExit("constraint violation before T")
+    cells = wl["n"][0] * wl["n"][1] * wl["n"][2]
+    return {
+        "metric": "wall time to T (whole run, one GPU)",
+        "value": wall,
+        "unit": "s",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "cells": cells, "end_time": T, "parallelism": "single GPU"},
+        "steps": int(p.steps),
+        "final_time": sim.get_current_time(),
+        "cell_updates_per_s": cells * p.steps / wall,
+        "tile_autotune": tune,
+    }
+
+
 def run_uq(args):
     import alsvinn_amd
     from alsvinn_amd import BatchRunner, generate_parameters
@@ -330,6 +359,9 @@ def main():
     ap.add_argument("--uq-to-end", action="store_true",
                     help="with --workload uq: run every sample to T = 2 and report the wall time, the Monte "
                          "Carlo errors of the statistics and the wall time to --uq-target-error")
+    ap.add_argument("--to-end", action="store_true",
+                    help="with --workload kh2d / kh3d: run the problem to its end time (T = 2 / 1) and report the "
+                         "wall time, the step count and the average rate")
     ap.add_argument("--uq-target-error", type=float, default=0.01,
                     help="relative Monte Carlo error of the mean and variance of rho (L1 over the domain)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -378,6 +410,9 @@ def main():
         shape, tune_ms = sim.autotune()
         tune = {"chosen": SHAPE_NAMES[dim][shape], "ms_per_shape": dict(zip(SHAPE_NAMES[dim], tune_ms))}
     sim.prepare_graph()  # capture + instantiate now: the timed steps only replay
+    if args.to_end:
+        print(json.dumps(run_to_end(sim, wl, tune)), flush=True)
+        return
 
     sim.enqueue_steps(args.warmup)
     sim.synchronize()

@@ -165,3 +165,57 @@ def test_mlp_full_shape_gradients():
                 sl = slice(off, off + size)
                 assert np.linalg.norm(g[m][sl] - go[sl]) <= 2e-2 * np.linalg.norm(go[sl]) + 1e-7
                 off += size
+
+
+def _conserved_totals(q):
+    """Sum over the cells of each conserved variable, compensated (math.fsum)."""
+    import math
+    return np.array([math.fsum(q[v].ravel()) for v in range(q.shape[0])])
+
+
+def _check_conservation(q0, q1, steps):
+    """Periodic boundaries: every conserved total stays what it was, up to
+    rounding, relative to the total of |values| of that variable (before or
+    after, whichever is larger).  The rounding drift grows linearly with the
+    step count, not with its square root: the RK3 weights 1/3 and 3/4 and the
+    dt/dx factor are rounded the same way in every cell, so each step scales
+    the totals by the same 1 + O(1e-16).  The oracle drifts 7.8e-17 per step
+    on KH 64^2 and 128^2; the bound is 4e-16 per step."""
+    t0, t1 = _conserved_totals(q0), _conserved_totals(q1)
+    scale = np.array([max(np.abs(q0[v]).sum(), np.abs(q1[v]).sum()) for v in range(q0.shape[0])])
+    drift = np.abs(t1 - t0) / scale
+    rel = 4e-16 * steps
+    _say("conserved totals", t0, "->", t1, "relative drift", drift)
+    assert np.all(drift <= rel), drift
+
+
+def test_kh2d_1024_to_T2_conserves_and_lands_on_T():
+    """The config-2 problem at its full size and end time (about 2.4e4
+    steps; the oracle would need hours, so the check is by property): the
+    last step lands on T = 2 exactly, the state stays physical, and mass,
+    momentum and energy totals are conserved to rounding."""
+    n = 1024
+    _say("case kh2d 1024^2 to T=2 (conservation)")
+    q0 = kelvin_helmholtz_2d(n, kh_params(2024))
+    h = alsvinn_amd.Simulator("euler2", "weno2", "hll3", "rungekutta3", "periodic", (n, n, 1), cfl=0.4)
+    h.set_interior(q0)
+    h.add_fixed_interval_writer(2.0)
+    p = h.run(2.0, chunk=512)
+    _say("steps", p.steps, "time", h.get_current_time())
+    assert h.get_current_time() == 2.0 and p.steps > 20000 and not p.violated
+    _check_conservation(q0, h.get_interior(), p.steps)
+
+
+def test_kh3d_256_to_T1_conserves_and_lands_on_T():
+    """The config-4 problem at its full size and end time (about 5.8e3
+    steps), by the same properties."""
+    n = 256
+    _say("case kh3d 256^3 to T=1 (conservation)")
+    q0 = kelvin_helmholtz_3d(n, kh_params(2024, count=80))
+    h = alsvinn_amd.Simulator("euler3", "weno2", "hll3", "rungekutta3", "periodic", (n, n, n), cfl=0.3)
+    h.set_interior(q0)
+    h.add_fixed_interval_writer(1.0)
+    p = h.run(1.0, chunk=256)
+    _say("steps", p.steps, "time", h.get_current_time())
+    assert h.get_current_time() == 1.0 and p.steps > 4000 and not p.violated
+    _check_conservation(q0, h.get_interior(), p.steps)
