@@ -137,7 +137,18 @@ __device__ __forceinline__ double draw_double(unsigned long long r, int elo, int
     return __longlong_as_double((long long)(((unsigned long long)(e + 1023) << 52) | mant));
 }
 
-enum { ARITH_SQRT = 0, ARITH_DIV_RHO = 1, ARITH_DIV_WENO = 2, ARITH_DIV_ROE = 3 };
+enum { ARITH_SQRT = 0, ARITH_DIV_RHO = 1, ARITH_DIV_WENO = 2, ARITH_DIV_ROE = 3, ARITH_FREXP = 4 };
+
+// frexp exponent of a finite double from its bits: 0 for +-0; for normal x
+// the biased exponent - 1022; for subnormal x = mant * 2^-1074 the position
+// of mant's leading bit (x = f * 2^e with 0.5 <= f < 1)
+__device__ __forceinline__ int frexp_exp_bits(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x) & 0x7fffffffffffffffull;
+    if (b == 0) return 0;
+    const int be = (int)(b >> 52);
+    if (be != 0) return be - 1022;
+    return (64 - __clzll((long long)b)) - 1074;
+}
 
 __global__ __launch_bounds__(256) void arith_selftest_kernel(int kind, unsigned long long seed, long long n,
                                                              unsigned long long* bad, double* first) {
@@ -152,7 +163,7 @@ __global__ __launch_bounds__(256) void arith_selftest_kernel(int kind, unsigned 
             want = sqrt(a);
         } else if (kind == ARITH_DIV_RHO) {
             // denominators in [2^-100, 2^100], numerators zero or +-[2^-800, 2^600]:
-            // the ranges of make_all_ops / div_rho
+            // the ranges of make_all_ops / sqrt_div_rho
             b = draw_double(r1, -100, 100);
             a = (r0 & 1023ull) == 0 ? 0.0 : draw_double(r0, -800, 600);
             if (r1 & (1ull << 63)) a = -a;
@@ -166,6 +177,22 @@ __global__ __launch_bounds__(256) void arith_selftest_kernel(int kind, unsigned 
             b = a0 + a1;
             got = div_shared(a, b, div_recip(b));
             want = a0 / (a0 + a1);
+        } else if (kind == ARITH_FREXP) {
+            // the exponent the range checks read (frexp_exp) on zeros,
+            // subnormals and normals of either sign: 1 draw in 8 a signed
+            // zero, 1 in 8 a subnormal, the rest normals over the whole range
+            const unsigned long long sel = r1 & 7ull;
+            if (sel == 0) {
+                a = (r1 & 8ull) ? -0.0 : 0.0;
+            } else if (sel == 1) {
+                const unsigned long long mant = (r0 & ((1ull << 52) - 1)) >> ((r1 >> 4) % 52ull);
+                a = __longlong_as_double((long long)(mant ? mant : 1ull));
+            } else {
+                a = draw_double(r0, -1022, 1023);
+            }
+            if (r1 & (1ull << 63)) a = -a;
+            got = (double)frexp_exp(a);
+            want = (double)frexp_exp_bits(a);
         } else {
             // Roe denominators sqrt(rhoL) + sqrt(rhoR) in [2^-50, 2^51],
             // numerators zero or +-[2^-800, 2^600]
@@ -1143,7 +1170,7 @@ avh_status avh_stats_read(avh_stats* t, int save, int primitive, double* mean, d
 avh_status avh_arith_selftest(avh_ctx* ctx, int kind, uint64_t seed, int64_t n, int64_t* mismatches,
                               double* first_bad) {
     if (!ctx || !mismatches) return fail(AVH_EINVAL, "null argument");
-    if (kind < ARITH_SQRT || kind > ARITH_DIV_ROE) return fail(AVH_EINVAL, "unknown self-test kind");
+    if (kind < ARITH_SQRT || kind > ARITH_FREXP) return fail(AVH_EINVAL, "unknown self-test kind");
     if (n < 0) return fail(AVH_EINVAL, "negative operand count");
     AVH_HIP(hipSetDevice(ctx->device));
     void* mem = nullptr;

@@ -53,15 +53,15 @@ __device__ __forceinline__ double div_shared(double n, double d, double r) {
     const double q0 = n * r;
     return fma(-fma(d, q0, -n), r, q0);
 }
-// n is zero or lo <= |n| <= hi (false for inf and NaN)
-__device__ __forceinline__ bool zero_or_within(double n, double lo, double hi) {
-    const double a = fabs(n);
-    return (a <= hi) & ((a >= lo) | (n == 0.0));
-}
 // lo <= n <= hi (false for zero, negative values, inf and NaN)
 __device__ __forceinline__ bool positive_within(double n, double lo, double hi) {
     return (n >= lo) & (n <= hi);
 }
+// Binary exponent e of x = f * 2^e, 0.5 <= |f| < 1 (v_frexp_exp_i32_f64):
+// 0 for +-0, the true exponent (<= -1022) for subnormals.  It is also 0 for
+// inf and NaN, so a check built on it must exclude those by another test.
+// `x is zero or |x| >= 2^(k-1)` is `frexp_exp(x) >= k`.
+__device__ __forceinline__ int frexp_exp(double x) { return __builtin_amdgcn_frexp_exp(x); }
 
 // Arithmetic of an interface's reconstruction and flux.
 //   PlainOps   the compiler's sqrt and divide everywhere.
@@ -154,10 +154,13 @@ struct EulerEq {
         return a;
     }
     // make_all under OPS.  SharedOps forms one reciprocal of rho for the
-    // divisions by rho; it checks that rho is in [2^-100, 2^100] and every
-    // momentum zero or in [2^-390, 2^290] in magnitude, so that the
-    // numerators 0.5*m.m and m[c] are zero or within [2^-781, 2^582] and
-    // every quotient within [2^-881, 2^682].
+    // divisions by rho.  It checks that rho is in [2^-100, 2^100], that
+    // m.m <= 2^580 (so every |m[c]| <= 2^290 (1 + 2^-54); false for inf and
+    // NaN) and, by binary exponent, that every momentum is zero or
+    // |m[c]| >= 2^-433.  Then the numerators m[c] are zero or within
+    // [2^-433, 2^291] and 0.5*m.m is zero or within [2^-867, 2^579] (the
+    // computed sum of squares is at least its largest term, 2^-866 or more,
+    // and nothing underflows), so every quotient lies within [2^-967, 2^679].
     template <class OPS>
     __device__ __forceinline__ static All make_all_ops(const double* q, double gamma, bool& ok) {
         if constexpr (!OPS::SHARED) {
@@ -168,11 +171,13 @@ struct EulerEq {
 #pragma unroll
             for (int c = 0; c < NSD_; ++c) a.m[c] = q[1 + c];
             a.E = q[NSD_ + 1];
-            ok = ok & positive_within(a.rho, 0x1p-100, 0x1p100);
+            const double mm = dot(a.m);
+            int emin = frexp_exp(a.m[0]);
 #pragma unroll
-            for (int c = 0; c < NSD_; ++c) ok = ok & zero_or_within(a.m[c], 0x1p-390, 0x1p290);
+            for (int c = 1; c < NSD_; ++c) emin = min(emin, frexp_exp(a.m[c]));
+            ok = ok & positive_within(a.rho, 0x1p-100, 0x1p100) & (mm <= 0x1p580) & (emin >= -432);
             const double r = div_recip(a.rho);
-            const double ie = a.E - div_shared(0.5 * dot(a.m), a.rho, r);
+            const double ie = a.E - div_shared(0.5 * mm, a.rho, r);
 #pragma unroll
             for (int c = 0; c < NSD_; ++c) a.u[c] = div_shared(a.m[c], a.rho, r);
             a.rinv = r;
@@ -180,33 +185,28 @@ struct EulerEq {
             return a;
         }
     }
-    // n / rho of a state from make_all_ops<OPS>; SharedOps checks that n is
-    // zero or within [2^-800, 2^600] (quotient within [2^-900, 2^700])
+    // sqrt(g*p / rho) for a state a from make_all_ops<OPS> (the sound speed;
+    // n = g*p).  SharedOps checks that n lies in [2^-600, 2^600], so that the
+    // quotient lies in [2^-700, 2^700], inside sqrt_core's range, and that
+    // gamma lies in [2^-20, 2^20] (div_roe's bounds on p rest on it).
     template <class OPS>
-    __device__ __forceinline__ static double div_rho(double n, const All& a, bool& ok) {
-        if constexpr (!OPS::SHARED) {
-            return n / a.rho;
-        } else {
-            ok = ok & zero_or_within(n, 0x1p-800, 0x1p600);
-            return div_shared(n, a.rho, a.rinv);
-        }
-    }
-    // sqrt(n / rho) for a state from make_all_ops<OPS>; SharedOps checks that
-    // n lies in [2^-600, 2^600], so that the quotient lies in [2^-700, 2^700],
-    // inside sqrt_core's range (the square root of a wave speed: n = gamma*p)
-    template <class OPS>
-    __device__ __forceinline__ static double sqrt_div_rho(double n, const All& a, bool& ok) {
+    __device__ __forceinline__ static double sqrt_div_rho(double n, double g, const All& a, bool& ok) {
         if constexpr (!OPS::SHARED) {
             return sqrt(n / a.rho);
         } else {
-            ok = ok & positive_within(n, 0x1p-600, 0x1p600);
+            ok = ok & positive_within(n, 0x1p-600, 0x1p600) & positive_within(g, 0x1p-20, 0x1p20);
             return sqrt_core(div_shared(n, a.rho, a.rinv));
         }
     }
     // (n1, n2) / (wl + wr) with wl, wr the square roots of the two sides'
-    // densities.  Under SharedOps the densities passed make_all_ops' check,
-    // so wl + wr is in [2^-50, 2^51]; the numerators are checked to be zero
-    // or within [2^-800, 2^600].
+    // densities: n1 = uL wl + uR wr, n2 = wl pL + wr pR.  Under SharedOps the
+    // densities passed make_all_ops' check, so wl + wr is in [2^-50, 2^51],
+    // and the interface's other checks bound the numerators: |u| <= 2^391
+    // (make_all_ops) and, with gamma in [2^-20, 2^20], p in [2^-620, 2^620]
+    // on both sides (sqrt_div_rho), so |n1| <= 2^442 and n2, a sum of two
+    // positive terms, lies in [2^-671, 2^671].  Only n1 can be small: it is checked, by
+    // binary exponent, to be zero or at least 2^-800 in magnitude (it is
+    // finite when the other checks pass).
     template <class OPS>
     __device__ __forceinline__ static void div_roe(double n1, double n2, double wl, double wr, double& q1,
                                                    double& q2, bool& ok) {
@@ -214,7 +214,7 @@ struct EulerEq {
             q1 = n1 / (wl + wr);
             q2 = n2 / (wl + wr);
         } else {
-            ok = ok & zero_or_within(n1, 0x1p-800, 0x1p600) & zero_or_within(n2, 0x1p-800, 0x1p600);
+            ok = ok & (frexp_exp(n1) >= -799);
             const double den = wl + wr;
             const double r = div_recip(den);
             q1 = div_shared(n1, den, r);
@@ -292,8 +292,8 @@ struct HLLFluxT {
         double ud, p;
         EQ::template div_roe<OPS>(L.u[DIR] * wl + R.u[DIR] * wr, L.p * wl + R.p * wr, wl, wr, ud, p, ok);
         const double cs = OPS::sqrt_(g * p / rho, ok);
-        const double cfl = EQ::template sqrt_div_rho<OPS>(g * L.p, L, ok);
-        const double cfr = EQ::template sqrt_div_rho<OPS>(g * R.p, R, ok);
+        const double cfl = EQ::template sqrt_div_rho<OPS>(g * L.p, g, L, ok);
+        const double cfr = EQ::template sqrt_div_rho<OPS>(g * R.p, g, R, ok);
         const double sl = fmin(L.u[DIR] - cfl, ud - cs);
         const double sr = fmax(R.u[DIR] + cfr, ud + cs);
         double fl[NV], fr[NV], ul[NV], ur[NV];
@@ -347,8 +347,8 @@ struct HLL3FluxT {
         const double rho = (L.rho + R.rho) / 2;
         double ud, p;
         EQ::template div_roe<OPS>(L.u[DIR] * wl + R.u[DIR] * wr, wl * L.p + wr * R.p, wl, wr, ud, p, ok);
-        const double cfl = EQ::template sqrt_div_rho<OPS>(g * L.p, L, ok);
-        const double cfr = EQ::template sqrt_div_rho<OPS>(g * R.p, R, ok);
+        const double cfl = EQ::template sqrt_div_rho<OPS>(g * L.p, g, L, ok);
+        const double cfr = EQ::template sqrt_div_rho<OPS>(g * R.p, g, R, ok);
         const double correct = 0.5 * fmax(0.0, L.u[DIR] - R.u[DIR]);
         const double cs = OPS::sqrt_(g * p / rho, ok);
         const double sl = fmin(L.u[DIR] + correct - cfl, ud - cs);

@@ -303,7 +303,9 @@ avh_status avh_substep(avh_ctx* ctx, const avh_config* cfg, const avh_volume* co
  * kind 0: sqrt over [2^-767, DBL_MAX]; 1: shared-reciprocal quotient with
  * denominators in [2^-100, 2^100] and numerators 0 or +-[2^-800, 2^600];
  * 2: WENO2 weight a0 / (a0 + a1), a0, a1 in [2^-400, 2^53];
- * 3: Roe quotient, denominators in [2^-50, 2^51].  Bounds are included.
+ * 3: Roe quotient, denominators in [2^-50, 2^51]; 4: the binary exponent
+ * the range checks read (v_frexp_exp_i32_f64) against the one the bits
+ * give, on signed zeros, subnormals and normals.  Bounds are included.
  * mismatches = count of
  * differing results, first_bad[2] (may be NULL) = operands of one of them.
  * No reference counterpart (the reference divides and takes square roots

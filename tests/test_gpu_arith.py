@@ -3,7 +3,9 @@ sequences where the operands admit it (sqrt without range fix-ups; one
 refined reciprocal shared by quotients with a common denominator).  These
 must give the bits of the compiler's sqrt and '/' on the device.  Each kind
 draws 10^8 operands over the range the kernels' guards admit, bounds
-included, and counts results that differ in any bit."""
+included, and counts results that differ in any bit.  The guards read
+binary exponents with the frexp instruction: kind 4 checks it against the
+exponent the bits give (0 for zeros, the true exponent for subnormals)."""
 import ctypes
 
 import pytest
@@ -17,7 +19,8 @@ pytestmark = pytest.mark.gpu
 KINDS = {0: "sqrt on [2^-767, DBL_MAX]",
          1: "n / rho, rho in [2^-100, 2^100], n 0 or +-[2^-800, 2^600]",
          2: "WENO2 weight a0 / (a0 + a1), a0, a1 in [2^-400, 2^53]",
-         3: "Roe quotient, denominator in [2^-50, 2^51]"}
+         3: "Roe quotient, denominator in [2^-50, 2^51]",
+        4: "binary exponent (frexp) of signed zeros, subnormals and normals"}
 
 
 @pytest.mark.parametrize("kind", sorted(KINDS))
