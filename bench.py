@@ -147,7 +147,7 @@ def cpu_baseline(wl, seconds_budget=15.0):
         sim.step()
         steps += 1
         el = time.perf_counter() - t0
-        if el >= seconds_budget or steps >= 1000:
+        if el >= seconds_budget or steps >= 20000:
             break
     cells = wl["n"][0] * wl["n"][1] * wl["n"][2]
     return {
@@ -347,8 +347,9 @@ def run_uq(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 600; uq: 100)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 30; uq: 5)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default kh3d 1500, kh2d 20000, uq 300: a few seconds of GPU work each)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 20; uq: 5)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=100)
@@ -387,9 +388,9 @@ def main():
         sys.exit(2)
 
     if args.steps is None:
-        args.steps = {"uq": 100, "kh3d": 100}.get(args.workload, 600)
+        args.steps = {"uq": 300, "kh3d": 1500, "kh2d": 20000}[args.workload]
     if args.warmup is None:
-        args.warmup = 5 if args.workload == "uq" else 10
+        args.warmup = 5 if args.workload == "uq" else 20
     if args.workload == "uq":
         print(json.dumps(run_uq(args)), flush=True)
         return
