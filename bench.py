@@ -59,9 +59,11 @@ def kernel_source_digest():
     return h.hexdigest()[:16]
 
 
-def load_profile(name):
-    """The profile summary `name` if it was measured on the current kernel
-    sources, else None (and the reason)."""
+def load_profile(name, shape, shape_names):
+    """The profile summary `name` for tile shape `shape` if it was measured
+    on the current kernel sources and that shape, else None (and the
+    reason).  A summary holds its main shape's figures at the top level and
+    those of further shapes under "other_shapes"."""
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
     if not os.path.exists(tfile):
         return None, "no profile summary %s" % name
@@ -69,7 +71,13 @@ def load_profile(name):
     if prof.get("kernel_source_digest") != kernel_source_digest():
         return None, "%s was measured on other kernel sources (digest %s, now %s)" % (
             name, prof.get("kernel_source_digest"), kernel_source_digest())
-    return prof, None
+    if prof.get("tile_shape", shape) == shape:
+        return prof, None
+    other = prof.get("other_shapes", {}).get(str(shape))
+    if other:
+        return other, None
+    return None, "%s was measured on tile shape %s, this run used %s" % (
+        name, shape_names[prof["tile_shape"]], shape_names[shape])
 
 
 def valu_bound(prof, avg_ms, source):
@@ -288,10 +296,7 @@ def run_uq(args):
     avg_bytes = cells * sum(sb) / len(sb)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
     pname = "batch_stage_kernel_uq_traffic.json"
-    prof, stale = load_profile(pname)
-    if prof and prof.get("tile_shape", shape) != shape:
-        prof, stale = None, "%s was measured on tile shape %s, this run used %s" % (
-            pname, SHAPE_NAMES[2][prof["tile_shape"]], SHAPE_NAMES[2][shape])
+    prof, stale = load_profile(pname, shape, SHAPE_NAMES[2])
     traffic = prof.get("bytes_per_launch") if prof else None
     valu = valu_bound(prof, avg_ms, pname)
     cpu = None
@@ -447,10 +452,7 @@ def main():
     traffic = None
     valu = None
     pname = "stage_kernel_%s_traffic.json" % WORKLOADS[args.workload]["profile"]
-    prof, stale = load_profile(pname)
-    if prof and prof.get("tile_shape", shape) != shape:
-        prof, stale = None, "%s was measured on tile shape %s, this run used %s" % (
-            pname, SHAPE_NAMES[dim][prof["tile_shape"]], SHAPE_NAMES[dim][shape])
+    prof, stale = load_profile(pname, shape, SHAPE_NAMES[dim])
     if prof:
         traffic = prof.get("bytes_per_launch")
         valu = valu_bound(prof, avg_ms, pname)
