@@ -330,6 +330,48 @@ template <bool FAST_>
 struct HLL3FluxT {
     typedef HLL3FluxT<false> Plain;
     static constexpr bool FAST = FAST_;  // the stage kernel runs flux_ops<SharedOps> first
+    // The star state of side k (k = L when s_m >= 0, else R): its speed sk,
+    // the star conserved state `mid`, the side's conserved state `uc` and
+    // its point flux `fk`, formed from side k's values by point_flux's
+    // operations (the same bits as selecting between both point fluxes).
+    // The transverse momentum fluxes in the star velocities are m[c] * u[DIR].
+    template <int DIR, class EQ>
+    __device__ __forceinline__ static void star_side(const typename EQ::All& L, const typename EQ::All& R, double sl,
+                                                     double sr, double& sk, double* mid, double* uc, double* fk) {
+        constexpr int NSD = EQ::NSD;
+        const double pl = L.p, pr = R.p;
+        const double udl = L.u[DIR] - sl;
+        const double udr = R.u[DIR] - sr;
+        const double aa = udr * R.rho - udl * L.rho;
+        const double sm = (R.m[DIR] * udr - L.m[DIR] * udl + pr - pl) / aa;
+
+        const bool left = sm >= 0;
+        const double rk = left ? L.rho : R.rho;
+        const double udk = left ? udl : udr;
+        sk = left ? sl : sr;
+        const double pk = left ? pl : pr;
+        const double uk = left ? L.u[DIR] : R.u[DIR];
+        const double ek = left ? L.E : R.E;
+
+        mid[0] = rk * udk / (sm - sk);
+#pragma unroll
+        for (int c = 0; c < NSD; ++c) {
+            // us = (fR.m - fL.m - sR*mR + sL*mL)/aa with us[DIR] = sm
+            const double us = (c == DIR) ? sm
+                                         : (R.m[c] * R.u[DIR] - L.m[c] * L.u[DIR] - R.m[c] * sr + L.m[c] * sl) / aa;
+            mid[1 + c] = us * mid[0];
+            uc[1 + c] = left ? L.m[c] : R.m[c];
+        }
+        const double pm = pk + rk * (uk - sm) * udk;
+        mid[NSD + 1] = (udk * ek + pk * uk - pm * sm) / (sm - sk);
+        uc[0] = rk;
+        uc[NSD + 1] = ek;
+        fk[0] = uc[1 + DIR];
+#pragma unroll
+        for (int c = 0; c < NSD; ++c) fk[1 + c] = uc[1 + c] * uk;
+        fk[1 + DIR] = fk[1 + DIR] + pk;
+        fk[NSD + 1] = (ek + pk) * uk;
+    }
     template <int DIR, class EQ>
     __device__ __forceinline__ static double flux(const typename EQ::All& L, const typename EQ::All& R,
                                                   double g, double* F) {
@@ -354,46 +396,21 @@ struct HLL3FluxT {
         const double sl = fmin(L.u[DIR] + correct - cfl, ud - cs);
         const double sr = fmax(R.u[DIR] - correct + cfr, ud + cs);
 
-        double fl[NV], fr[NV];
-        EQ::template point_flux<DIR>(L, fl);
-        EQ::template point_flux<DIR>(R, fr);
-        const double pl = L.p, pr = R.p;
-        const double udl = L.u[DIR] - sl;
-        const double udr = R.u[DIR] - sr;
-        const double aa = udr * R.rho - udl * L.rho;
-        const double sm = (R.m[DIR] * udr - L.m[DIR] * udl + pr - pl) / aa;
-
-        const bool left = sm >= 0;
-        const double rk = left ? L.rho : R.rho;
-        const double udk = left ? udl : udr;
-        const double sk = left ? sl : sr;
-        const double pk = left ? pl : pr;
-        const double uk = left ? L.u[DIR] : R.u[DIR];
-        const double ek = left ? L.E : R.E;
-
-        double mid[NV], uc[NV], fk[NV];
-        mid[0] = rk * udk / (sm - sk);
-#pragma unroll
-        for (int c = 0; c < NSD; ++c) {
-            // us = (fR.m - fL.m - sR*mR + sL*mL)/aa with us[DIR] = sm
-            const double us =
-                (c == DIR) ? sm : (fr[1 + c] - fl[1 + c] - R.m[c] * sr + L.m[c] * sl) / aa;
-            mid[1 + c] = us * mid[0];
-            uc[1 + c] = left ? L.m[c] : R.m[c];
-        }
-        const double pm = pk + rk * (uk - sm) * udk;
-        mid[NSD + 1] = (udk * ek + pk * uk - pm * sm) / (sm - sk);
-        uc[0] = rk;
-        uc[NSD + 1] = ek;
-#pragma unroll
-        for (int i = 0; i < NV; ++i) fk[i] = left ? fl[i] : fr[i];
-        // subsonic interfaces (sL < 0 < sR) take the star branch; when the
-        // whole wavefront is subsonic the other three branches are skipped
+        // subsonic interfaces (sL < 0 < sR) take the star branch.  Under
+        // SharedOps a wavefront that is subsonic throughout skips the other
+        // three branches and never forms both point fluxes (fewer values
+        // live across the star formulas); otherwise both point fluxes are
+        // formed first, as the general branch selects between them.
         const bool star = !(sl == 0) && !(sl > 0) && !(sr < 0);
+        double sk, mid[NV], uc[NV], fk[NV];
+        star_side<DIR, EQ>(L, R, sl, sr, sk, mid, uc, fk);
         if (__all(star)) {
 #pragma unroll
             for (int i = 0; i < NV; ++i) F[i] = fk[i] + sk * (mid[i] - uc[i]);
         } else {
+            double fl[NV], fr[NV];
+            EQ::template point_flux<DIR>(L, fl);
+            EQ::template point_flux<DIR>(R, fr);
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
                 double v = fk[i] + sk * (mid[i] - uc[i]);

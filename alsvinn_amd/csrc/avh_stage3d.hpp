@@ -98,6 +98,40 @@ __device__ __forceinline__ void recon_face_reg(const double (&u)[WZ][NV], const 
     recon_face_reg<RECON, NV, RIGHT, WZ, PlainOps>(u, c, out, ok);
 }
 
+// Both faces of the window's cell c.  WENO2 under SharedOps divides once per
+// denominator: the left face takes a0 = 1/d0, a1 = 2/d1 and the right face
+// a0 = 2/d0, a1 = 1/d1, and 2/d is formed as 2 * (1/d).  Doubling is exact
+// and commutes with rounding while 2/d stays normal; the weight checks
+// (every a0, a1 >= 2^-400, so 1/d0, 1/d1 >= 2^-400 and d0, d1 <= 2^400) hold
+// only there, so a pair that passes them has the bits of the two divisions.
+template <int RECON, int NV, int WZ, class OPS>
+__device__ __forceinline__ void recon_faces_reg(const double (&u)[WZ][NV], const int c, double* outl,
+                                                double* outr, bool& ok) {
+    if constexpr (RECON == RECON_WENO2 && OPS::SHARED) {
+        const double eps = 1e-8;
+        const double w0 = u[c][0];
+        const double dp = u[c + 1][0] - w0;
+        const double b0 = dp * dp;
+        const double dm = w0 - u[c - 1][0];
+        const double b1 = dm * dm;
+        const double d0 = 3 * (eps + b0) * (eps + b0);
+        const double d1 = 3 * (eps + b1) * (eps + b1);
+        const double r0 = 1 / d0;
+        const double r1 = 1 / d1;
+        double w0l, w1l, w0r, w1r;
+        weno2_weights<OPS>(r0, 2 * r1, w0l, w1l, ok);
+        weno2_weights<OPS>(2 * r0, r1, w0r, w1r, ok);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            outl[v] = 0.5 * (w1l * u[c - 1][v] + (3 * w0l + w1l) * u[c][v] - w0l * u[c + 1][v]);
+            outr[v] = 0.5 * (w0r * u[c + 1][v] + (3 * w1r + w0r) * u[c][v] - w1r * u[c - 1][v]);
+        }
+    } else {
+        recon_face_reg<RECON, NV, false, WZ, OPS>(u, c, outl, ok);
+        recon_face_reg<RECON, NV, true, WZ, OPS>(u, c, outr, ok);
+    }
+}
+
 // TWO_BARRIERS: the plane tile is double-buffered and the x- and y-interface
 // fluxes have LDS buffers of their own, so a plane takes two barriers (plane
 // published; fluxes published) instead of five, and the x, y and z flux
@@ -383,8 +417,7 @@ struct MarchKernel3 {
                     typedef typename OpsFor<FLUX::FAST>::type Ops;
                     double lz[NV];
                     bool ok = true;
-                    recon_face_reg<RECON, NV, false, WZ, Ops>(w, HW, lz, ok);
-                    recon_face_reg<RECON, NV, true, WZ, Ops>(w, HW, rnext, ok);
+                    recon_faces_reg<RECON, NV, WZ, Ops>(w, HW, lz, rnext, ok);
                     typename EQ::All aL = EQ::template make_all_ops<Ops>(rprev, P.gamma, ok);
                     typename EQ::All aR = EQ::template make_all_ops<Ops>(lz, P.gamma, ok);
                     double wz = FLUX::template flux_ops<Ops, 2, EQ>(aL, aR, P.gamma, fz, ok);
