@@ -121,6 +121,7 @@ struct BurgersEq {
         const double u = q[0];
         return !__builtin_isnan(u) && !__builtin_isinf(u);
     }
+    __device__ __forceinline__ static bool obeys_fast(const double* q, double g) { return obeys(q, g); }
 };
 
 template <int NSD_>
@@ -241,6 +242,23 @@ struct EulerEq {
 #pragma unroll
         for (int c = 0; c < NSD_; ++c) ok = ok && !__builtin_isnan(a.u[c]) && !__builtin_isinf(a.u[c]);
         return ok && (a.rho > 0) && (a.p > 0);
+    }
+    // obeys() with computeExtra's quotients through one reciprocal of rho
+    // (make_all_ops<SharedOps>) where the state lies in that range, which
+    // gives the same extra variables and so the same answer; any other
+    // state takes obeys() itself.
+    __device__ __forceinline__ static bool obeys_fast(const double* q, double gamma) {
+        bool in_range = true;
+        const All a = make_all_ops<SharedOps>(q, gamma, in_range);
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) ok = ok & !__builtin_isnan(q[i]) & !__builtin_isinf(q[i]);
+        ok = ok & !__builtin_isnan(a.p) & !__builtin_isinf(a.p);
+#pragma unroll
+        for (int c = 0; c < NSD_; ++c) ok = ok & !__builtin_isnan(a.u[c]) & !__builtin_isinf(a.u[c]);
+        ok = ok & (a.rho > 0) & (a.p > 0);
+        if (!in_range) ok = obeys(q, gamma);
+        return ok;
     }
 };
 

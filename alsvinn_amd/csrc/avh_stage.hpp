@@ -457,7 +457,7 @@ struct StageKernel {
                     r[v] = o;
                 }
             }
-            if (final_stage) bad = bad || !EQ::obeys(r, P.gamma);
+            if (final_stage) bad = bad || !EQ::obeys_fast(r, P.gamma);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 if constexpr (VOL) V->out[v][gi] = r[v];

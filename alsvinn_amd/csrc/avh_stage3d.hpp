@@ -291,14 +291,14 @@ struct MarchKernel3 {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
             fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
-            {
+            if (col_ok) {
                 double l0[NV];
                 recon_face_reg<RECON, NV, false, WZ>(w, HW, l0);
                 recon_face_reg<RECON, NV, true, WZ>(w, HW, rprev);
                 const typename EQ::All aL = EQ::make_all(rm1, P.gamma);
                 const typename EQ::All aR = EQ::make_all(l0, P.gamma);
                 const double wz = FLUX::template flux<2, EQ>(aL, aR, P.gamma, fprev);
-                if (col_ok) ws[2] = ws_max(ws[2], wz);
+                ws[2] = ws_max(ws[2], wz);
             }
 #pragma unroll
             for (int i = 0; i < WZ - 1; ++i)
@@ -412,8 +412,10 @@ struct MarchKernel3 {
             double fz[NV], rnext[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) fz[v] = rnext[v] = 0.0;
+            // (only for the tile's columns inside the grid: a column past the
+            // grid edge supplies the plane's halo but has no cells to update)
             auto z_flux = [&]() {
-                if (has_col) {
+                if (col_ok) {
                     typedef typename OpsFor<FLUX::FAST>::type Ops;
                     double lz[NV];
                     bool ok = true;
@@ -430,7 +432,7 @@ struct MarchKernel3 {
                         bool ok2 = true;
                         wz = FLUX::template flux_ops<PlainOps, 2, EQ>(aL, aR, P.gamma, fz, ok2);
                     }
-                    if (col_ok) ws[2] = ws_max(ws[2], wz);
+                    ws[2] = ws_max(ws[2], wz);
                 }
             };
             auto add_x = [&]() {
@@ -488,7 +490,7 @@ struct MarchKernel3 {
                         r[v] = o;
                     }
                 }
-                if (final_stage) bad = bad || !EQ::obeys(r, P.gamma);
+                if (final_stage) bad = bad || !EQ::obeys_fast(r, P.gamma);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) P.out[v * P.ncells + gi] = r[v];
             }
