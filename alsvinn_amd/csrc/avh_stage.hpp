@@ -164,7 +164,9 @@ __device__ __forceinline__ double step_dt(const Control* c) {
     return dt;
 }
 
-template <class EQ, int RECON, class FLUX, int DIM, int TX, int TY, int TZ, int NT>
+// MODE_CT >= 0 compiles the kernel for that StageMode alone (P.mode must
+// equal it): the other modes' branches and registers drop out.
+template <class EQ, int RECON, class FLUX, int DIM, int TX, int TY, int TZ, int NT, int MODE_CT = -1>
 struct StageKernel {
     static constexpr int NV = EQ::NV;
     static constexpr int G = ReconTraits<RECON>::GHOST;
@@ -317,7 +319,8 @@ struct StageKernel {
         double* sf = lds + NV * LT;
 
         Control* ctrl = P.ctrl;
-        if (!ctrl->active && P.mode != MODE_FLUX_ONLY) return;  // step past the end: no-op
+        const int mode = MODE_CT >= 0 ? MODE_CT : P.mode;
+        if (!ctrl->active && mode != MODE_FLUX_ONLY) return;  // step past the end: no-op
 
         const int n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
         const int ntx = (n0 + TX - 1) / TX, nty = (n1 + TY - 1) / TY;
@@ -326,8 +329,8 @@ struct StageKernel {
         const int tz = tile / (ntx * nty);
         const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
         const long long nc = P.ncells;
-        const bool otf = (P.mode == MODE_RK3_2);
-        const double s = (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) ? 0.0 : step_dt(ctrl) / ctrl->cell[0];
+        const bool otf = (mode == MODE_RK3_2);
+        const double s = (mode == MODE_FLUX_ONLY || mode == MODE_L) ? 0.0 : step_dt(ctrl) / ctrl->cell[0];
 
         // 1. stage input tile + halo into LDS through the boundary map.  Each
         // thread issues the loads of all its points before writing any of
@@ -406,8 +409,8 @@ struct StageKernel {
         // cells are issued first (cells outside the grid load element 0 and
         // write nothing)
         bool bad = false;
-        const bool final_stage = (P.mode == MODE_RK2_2 || P.mode == MODE_RK3_3);
-        const bool combine = !(P.mode == MODE_FLUX_ONLY || P.mode == MODE_L);
+        const bool final_stage = (mode == MODE_RK2_2 || mode == MODE_RK3_3);
+        const bool combine = !(mode == MODE_FLUX_ONLY || mode == MODE_L);
         long long gc[CPT];
         bool inside[CPT];
 #pragma unroll
@@ -421,8 +424,8 @@ struct StageKernel {
             if constexpr (VOL) gc[m] = inside[m] ? volume_index(*V, gx, gy, gz) : 0;
             else gc[m] = inside[m] ? ((long long)gz * n1 + gy) * n0 + gx : 0;
         }
-        const double ca = P.mode == MODE_RK3_2 ? 1. / 3. : (P.mode == MODE_RK3_3 ? 2.0 : 1.0);
-        const double cb = P.mode == MODE_RK2_2 ? 0.5 : (P.mode == MODE_RK3_2 ? 3. / 4. : 1. / 3.);
+        const double ca = mode == MODE_RK3_2 ? 1. / 3. : (mode == MODE_RK3_3 ? 2.0 : 1.0);
+        const double cb = mode == MODE_RK2_2 ? 0.5 : (mode == MODE_RK3_2 ? 3. / 4. : 1. / 3.);
         double u0c[CPT][NV];
         if (combine) {
 #pragma unroll
@@ -465,7 +468,7 @@ struct StageKernel {
             }
         }
 
-        if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
+        if (mode == MODE_FLUX_ONLY || mode == MODE_L) {
             // wave speed: wavefront reduction, then one atomicMax per direction
             // per workgroup (non-negative doubles order like their bit patterns)
             const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;

@@ -136,7 +136,9 @@ __device__ __forceinline__ void recon_faces_reg(const double (&u)[WZ][NV], const
 // fluxes have LDS buffers of their own, so a plane takes two barriers (plane
 // published; fluxes published) instead of five, and the x, y and z flux
 // evaluations of a plane run in one barrier-free stretch.
-template <class EQ, int RECON, class FLUX, int TX, int TY, int LZ, int NTH, bool TWO_BARRIERS = false>
+// MODE_CT >= 0 compiles the kernel for that StageMode alone (P.mode must
+// equal it): the other modes' branches and registers drop out.
+template <class EQ, int RECON, class FLUX, int TX, int TY, int LZ, int NTH, bool TWO_BARRIERS = false, int MODE_CT = -1>
 struct MarchKernel3 {
     static constexpr int NV = EQ::NV;
     static constexpr int NC = TX * TY;  // columns: one per thread, threads past them work on the planes only
@@ -244,7 +246,8 @@ struct MarchKernel3 {
         double* const sfy = TWO_BARRIERS ? sf_y : sf;
 
         Control* ctrl = P.ctrl;
-        if (!ctrl->active && P.mode != MODE_FLUX_ONLY) return;
+        const int mode = MODE_CT >= 0 ? MODE_CT : P.mode;
+        if (!ctrl->active && mode != MODE_FLUX_ONLY) return;
 
         const int n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
         const int ntx = (n0 + TX - 1) / TX, nty = (n1 + TY - 1) / TY;
@@ -254,8 +257,8 @@ struct MarchKernel3 {
         const int tz = tile / (ntx * nty);
         const int ox = tx * TX, oy = ty * TY, z0 = tz * LZ;
         const int z1 = (z0 + LZ < n2) ? z0 + LZ : n2;
-        const bool otf = (P.mode == MODE_RK3_2);
-        const double s = (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) ? 0.0 : step_dt(ctrl) / ctrl->cell[0];
+        const bool otf = (mode == MODE_RK3_2);
+        const double s = (mode == MODE_FLUX_ONLY || mode == MODE_L) ? 0.0 : step_dt(ctrl) / ctrl->cell[0];
 
         // this thread's column.  A column past the grid edge (partial tile)
         // sits where the tile's x/y halo lies, so it loads the boundary image
@@ -308,7 +311,7 @@ struct MarchKernel3 {
         }
 
         bool bad = false;
-        const bool final_stage = (P.mode == MODE_RK2_2 || P.mode == MODE_RK3_3);
+        const bool final_stage = (mode == MODE_RK2_2 || mode == MODE_RK3_3);
         // This thread's cells of the x/y halo ring (HPT per thread at most):
         // their LDS slots and boundary-mapped positions in a plane do not
         // change with k.  The ring rows above and below the tile come first,
@@ -369,9 +372,9 @@ struct MarchKernel3 {
         // RK combination ((L*s + U) * ca + U0) * cb (the factor 1 of the RK2
         // stage is exact); U0 of the column's next plane is loaded one plane
         // ahead, with the window's next plane
-        const bool combine = !(P.mode == MODE_FLUX_ONLY || P.mode == MODE_L);
-        const double ca = P.mode == MODE_RK3_2 ? 1. / 3. : (P.mode == MODE_RK3_3 ? 2.0 : 1.0);
-        const double cb = P.mode == MODE_RK2_2 ? 0.5 : (P.mode == MODE_RK3_2 ? 3. / 4. : 1. / 3.);
+        const bool combine = !(mode == MODE_FLUX_ONLY || mode == MODE_L);
+        const double ca = mode == MODE_RK3_2 ? 1. / 3. : (mode == MODE_RK3_3 ? 2.0 : 1.0);
+        const double cb = mode == MODE_RK2_2 ? 0.5 : (mode == MODE_RK3_2 ? 3. / 4. : 1. / 3.);
         double u0n[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) u0n[v] = 0.0;
@@ -532,7 +535,7 @@ struct MarchKernel3 {
         }
 
 #undef AVH_PREFETCH_RING
-        if (P.mode == MODE_FLUX_ONLY || P.mode == MODE_L) {
+        if (mode == MODE_FLUX_ONLY || mode == MODE_L) {
             const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {

@@ -55,6 +55,13 @@ template <class EQ, int RECON, class FLUX, int S>
 struct KernelFor<EQ, RECON, FLUX, 2, S> {
     typedef StageKernel<EQ, RECON, FLUX, 2, SHAPE2_TX[S], SHAPE2_TY[S], 1, 256> type;
 };
+// 2D shapes whose RK3 stages are each compiled for their mode (the shapes
+// the autotuner takes for the benchmark grids and the batched set)
+constexpr bool SHAPE2_BY_MODE[NSHAPES2] = {false, false, false, true, true};
+template <class EQ, int RECON, class FLUX, int S, int MODE>
+struct KernelForMode {
+    typedef StageKernel<EQ, RECON, FLUX, 2, SHAPE2_TX[S], SHAPE2_TY[S], 1, 256, MODE> type;
+};
 
 // stage tiles (workgroups) covering one grid of the 1D / 2D kernels
 inline unsigned int stage_tiles(const int* n, int dim, int shape) {
@@ -80,13 +87,28 @@ hipError_t launch_batch_stage(const BatchParams& B, unsigned int nblocks, int sh
         }
         return hipErrorInvalidValue;
     } else {
+#define AVH_BATCH_MODES(S)                                                                                  \
+    case S:                                                                                                 \
+        if constexpr (SHAPE2_BY_MODE[S]) {                                                                  \
+            if (B.mode == MODE_L)                                                                           \
+                return launch_batch_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_L>::type>(B, nblocks, \
+                                                                                               stream);     \
+            if (B.mode == MODE_RK3_2)                                                                       \
+                return launch_batch_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_RK3_2>::type>(        \
+                    B, nblocks, stream);                                                                    \
+            if (B.mode == MODE_RK3_3)                                                                       \
+                return launch_batch_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_RK3_3>::type>(        \
+                    B, nblocks, stream);                                                                    \
+        }                                                                                                   \
+        return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, S>::type>(B, nblocks, stream);
         switch (shape) {
-        case 0: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 0>::type>(B, nblocks, stream);
-        case 1: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 1>::type>(B, nblocks, stream);
-        case 2: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 2>::type>(B, nblocks, stream);
-        case 3: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 3>::type>(B, nblocks, stream);
-        case 4: return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, 4>::type>(B, nblocks, stream);
+            AVH_BATCH_MODES(0)
+            AVH_BATCH_MODES(1)
+            AVH_BATCH_MODES(2)
+            AVH_BATCH_MODES(3)
+            AVH_BATCH_MODES(4)
         }
+#undef AVH_BATCH_MODES
         return hipErrorInvalidValue;
     }
 }
@@ -121,7 +143,22 @@ hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
             AVH_MARCH_CASE(3)
             AVH_MARCH_CASE(4)
             AVH_MARCH_CASE(5)
-            AVH_MARCH_CASE(6)
+        case 6: {
+            // the RK3 stages of the shape a 3D run normally takes, each compiled
+            // for its mode; other modes take the kernel that reads P.mode
+            typedef typename std::conditional<SHAPE3_SHARED[6], FLUX, typename FLUX::Plain>::type F6;
+            if (P.mode == MODE_L)
+                return launch_march_k<MarchKernel3<EQ, RECON, F6, SHAPE3_TX[6], SHAPE3_TY[6], SHAPE3_LZ[6], 256,
+                                                   SHAPE3_TWO_BARRIERS[6], MODE_L>>(P, stream);
+            if (P.mode == MODE_RK3_2)
+                return launch_march_k<MarchKernel3<EQ, RECON, F6, SHAPE3_TX[6], SHAPE3_TY[6], SHAPE3_LZ[6], 256,
+                                                   SHAPE3_TWO_BARRIERS[6], MODE_RK3_2>>(P, stream);
+            if (P.mode == MODE_RK3_3)
+                return launch_march_k<MarchKernel3<EQ, RECON, F6, SHAPE3_TX[6], SHAPE3_TY[6], SHAPE3_LZ[6], 256,
+                                                   SHAPE3_TWO_BARRIERS[6], MODE_RK3_3>>(P, stream);
+            return launch_march_k<MarchKernel3<EQ, RECON, F6, SHAPE3_TX[6], SHAPE3_TY[6], SHAPE3_LZ[6], 256,
+                                               SHAPE3_TWO_BARRIERS[6]>>(P, stream);
+        }
         }
 #undef AVH_MARCH_CASE
         return hipErrorInvalidValue;
@@ -132,13 +169,27 @@ hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
         }
         return hipErrorInvalidValue;
     } else {
+#define AVH_STAGE_MODES(S)                                                                                  \
+    case S:                                                                                                 \
+        if constexpr (SHAPE2_BY_MODE[S]) {                                                                  \
+            if (P.mode == MODE_L)                                                                           \
+                return launch_stage_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_L>::type>(P, stream); \
+            if (P.mode == MODE_RK3_2)                                                                       \
+                return launch_stage_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_RK3_2>::type>(P,      \
+                                                                                                 stream);   \
+            if (P.mode == MODE_RK3_3)                                                                       \
+                return launch_stage_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_RK3_3>::type>(P,      \
+                                                                                                 stream);   \
+        }                                                                                                   \
+        return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, S>::type>(P, stream);
         switch (shape) {
-        case 0: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 0>::type>(P, stream);
-        case 1: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 1>::type>(P, stream);
-        case 2: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 2>::type>(P, stream);
-        case 3: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 3>::type>(P, stream);
-        case 4: return launch_stage_k<typename KernelFor<EQ, RECON, FLUX, 2, 4>::type>(P, stream);
+            AVH_STAGE_MODES(0)
+            AVH_STAGE_MODES(1)
+            AVH_STAGE_MODES(2)
+            AVH_STAGE_MODES(3)
+            AVH_STAGE_MODES(4)
         }
+#undef AVH_STAGE_MODES
         return hipErrorInvalidValue;
     }
 }
