@@ -163,15 +163,32 @@ struct MarchKernel3 {
         return ((long long)z * n1 + y) * n0 + x;
     }
 
-    // one input value at a (boundary-mapped) global cell
-    __device__ __forceinline__ static void fetch(const StageParams& P, bool otf, double s, long long gi, double* q) {
+    // one input value at a (boundary-mapped) global cell; in the on-the-fly
+    // stage U0's values are also written to `keep` (stride NC) when given
+    __device__ __forceinline__ static void fetch(const StageParams& P, bool otf, double s, long long gi, double* q,
+                                                 double* keep = nullptr) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             double val = P.in[v * P.ncells + gi];
-            if (otf) val = val * s + P.u0[v * P.ncells + gi];
+            if (otf) {
+                const double u = P.u0[v * P.ncells + gi];
+                if (keep) keep[v * NC] = u;
+                val = val * s + u;
+            }
             q[v] = val;
         }
     }
+
+    // The kernel compiled for the on-the-fly stage (U1 = L0*s + U0 formed as
+    // the tile loads) reads the column's U0 with each window plane; its RK
+    // combination needs the same U0 again RING planes later, by which time
+    // the line has left L2.  A thread-private LDS ring of RING planes keeps
+    // it, where LDS still allows two workgroups per CU (the register budget
+    // allows no more).
+    static constexpr int RING = 2 + HW;
+    static constexpr int LDS_BASE =
+        8 * ((TWO_BARRIERS ? 2 : 1) * NV * LP + (TWO_BARRIERS ? 2 : 1) * NV * NIMAX + 3 * (NT / 64));
+    static constexpr bool U0RING = MODE_CT == MODE_RK3_2 && LDS_BASE + 8 * RING * NV * NC <= 80 * 1024;
 
     // interface between plane cells ia and ia + SD under OPS
     template <class OPS, int DIR>
@@ -242,6 +259,7 @@ struct MarchKernel3 {
         __shared__ double sf[NV * NIMAX];
         __shared__ double sf_y[TWO_BARRIERS ? NV * NIMAX : 1];
         __shared__ double s_ws[3][NT / 64];
+        __shared__ double su0[U0RING ? RING * NV * NC : 1];
         double* const sfx = sf;
         double* const sfy = TWO_BARRIERS ? sf_y : sf;
 
@@ -281,11 +299,15 @@ struct MarchKernel3 {
 
         // prologue: window centred at z0-1 -> R(z0-1); then centred at z0 ->
         // L(z0), R(z0), F_z(z0-1/2); then centred at z0+1 for the loop
+        // ring slot of the column's plane p (z0 <= p): U0 at (p - z0) % RING
+        const bool ring = U0RING && col_ok;
+        double* const su0c = su0 + (U0RING ? threadIdx.x : 0);
+        auto slot = [&](int p) { return su0c + ((p - z0) % RING) * NV * NC; };
         if (has_col) {
 #pragma unroll
             for (int i = 0; i < WZ; ++i) {
                 const int gz = bc_map(z0 - 1 - HW + i, n2, P.bc);
-                fetch(P, otf, s, gidx(gxc, gyc, gz, n0, n1), w[i]);
+                fetch(P, otf, s, gidx(gxc, gyc, gz, n0, n1), w[i], (ring && i > HW) ? slot(z0 - 1 - HW + i) : nullptr);
             }
             double rm1[NV];
             recon_face_reg<RECON, NV, true, WZ>(w, HW, rm1);
@@ -293,7 +315,8 @@ struct MarchKernel3 {
             for (int i = 0; i < WZ - 1; ++i)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
-            fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
+            fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + HW, n2, P.bc), n0, n1), w[WZ - 1],
+                  ring ? slot(z0 + HW) : nullptr);
             if (col_ok) {
                 double l0[NV];
                 recon_face_reg<RECON, NV, false, WZ>(w, HW, l0);
@@ -307,7 +330,8 @@ struct MarchKernel3 {
             for (int i = 0; i < WZ - 1; ++i)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) w[i][v] = w[i + 1][v];
-            fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + 1 + HW, n2, P.bc), n0, n1), w[WZ - 1]);
+            fetch(P, otf, s, gidx(gxc, gyc, bc_map(z0 + 1 + HW, n2, P.bc), n0, n1), w[WZ - 1],
+                  ring ? slot(z0 + 1 + HW) : nullptr);
         }
 
         bool bad = false;
@@ -378,7 +402,7 @@ struct MarchKernel3 {
         double u0n[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) u0n[v] = 0.0;
-        if (combine && col_ok) {
+        if (combine && col_ok && !ring) {
             const long long gi = gidx(ox + cx, oy + cy, z0, n0, n1);
 #pragma unroll
             for (int v = 0; v < NV; ++v) u0n[v] = P.u0[v * P.ncells + gi];
@@ -478,6 +502,11 @@ struct MarchKernel3 {
 
             if (col_ok) {
                 const long long gi = gidx(ox + cx, oy + cy, k, n0, n1);
+                if (ring) {
+                    const double* const u0k = slot(k);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) u0n[v] = u0k[v * NC];
+                }
                 double r[NV];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
@@ -522,9 +551,14 @@ struct MarchKernel3 {
 #pragma unroll
                     for (int v = 0; v < NV; ++v) cu0[v] = P.u0[v * P.ncells + gn];
                 }
-                if (combine && col_ok) {
+                if (combine && col_ok && !ring) {
 #pragma unroll
                     for (int v = 0; v < NV; ++v) u0n[v] = P.u0[v * P.ncells + gw];
+                }
+                if (ring && k + 2 + HW < z1) {
+                    double* const kp = slot(k + 2 + HW);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) kp[v * NC] = cu0[v];
                 }
 #pragma unroll
                 for (int v = 0; v < NV; ++v) w[WZ - 1][v] = otf ? cin[v] * s + cu0[v] : cin[v];

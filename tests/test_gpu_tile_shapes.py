@@ -22,6 +22,10 @@ CASES = [
     ("burgers", "none", "godunov", "rungekutta3", "periodic", (33, 66, 1), 0.4, 5),
     ("euler3", "weno2", "hll3", "rungekutta3", "periodic", (35, 19, 40), 0.3, 7),
     ("euler3", "eno3", "hll", "rungekutta2", "neumann", (17, 33, 36), 0.3, 7),
+    # RK3 on every 3D shape for the other reconstructions (the last shape's
+    # stages run kernels compiled per stage mode, stage 2 with its U0 ring)
+    ("euler3", "eno2", "hll", "rungekutta3", "neumann", (23, 17, 37), 0.3, 7),
+    ("euler3", "none", "hll3", "rungekutta3", "periodic", (18, 31, 33), 0.3, 7),
 ]
 
 
