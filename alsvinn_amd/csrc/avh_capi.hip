@@ -34,33 +34,12 @@ using namespace avh;
 
 namespace {
 
-// End of a step, one thread: t += dt, constraint flag, then the writers'
-// save counters (alsfvm/src/simulator/Simulator.cpp:81-85, 180-214;
-// alsfvm/src/io/FixedIntervalWriter.cpp:33-43) and the loop test
-// `t >= endTime` (Simulator.cpp:77-79); clears the wave-speed maxima for the
-// next step.
+// End of a step (end_step) as its own launch, after forward euler's
+// element-wise pass; the RK2 and RK3 final stages end the step themselves
+// (StageParams::finish).
 __global__ __launch_bounds__(64) void finish_step_kernel(Control* c) {
     if (threadIdx.x != 0 || !c->active) return;
-    const double dt = step_dt(c);
-    c->dt = dt;
-    c->scale = dt / c->cell[0];
-    c->t = c->t + dt;
-    c->steps = c->steps + 1;
-    c->last_dt = dt;
-    if (c->block_violation) {
-        c->violated = 1;
-    } else {
-        for (int i = 0; i < c->n_adj; ++i) {
-            if (c->t >= (double)c->adj_saved[i] * c->adj_interval[i] + c->adj_start[i]) {
-                c->adj_saved[i] = c->adj_saved[i] + 1;
-                c->saves_pending = c->saves_pending + 1;
-            }
-        }
-    }
-    c->block_violation = 0;
-    clear_wave_speeds(c);
-    const bool more = !(c->t >= c->t_end) && !c->violated && !(c->pause_on_save && c->saves_pending > 0);
-    c->active = more ? 1 : 0;
+    end_step(c, c->block_violation != 0);
 }
 
 enum EwMode { EW_FE_FINISH = 0, EW_RK2_U1 = 1 };
@@ -406,6 +385,7 @@ static __global__ void noalloc_init_kernel(Control* c, double c0, double c1, dou
     c->steps = 0;
     c->active = 1;
     c->violated = 0;
+    c->arrive = 0ull;
     c->saves_pending = 0;
     c->pause_on_save = 0;
     c->block_violation = 0;
@@ -477,6 +457,9 @@ static avh_status enqueue_step(avh_solver* s, EventMarks* tm = nullptr) {
         s->ew(s->B, s->A, s->A, s->ctrl, s->ncells, EW_FE_FINISH, s->cfg.gamma, s->stream);
         AVH_HIP(hipGetLastError());
         if ((r = mark(s, tm))) return r;
+        hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s->stream, s->ctrl);
+        AVH_HIP(hipGetLastError());
+        if ((r = mark(s, tm))) return r;
     } else if (s->cfg.integrator == AVH_INT_RUNGEKUTTA2) {
         s->ew(s->B, s->A, s->C, s->ctrl, s->ncells, EW_RK2_U1, s->cfg.gamma, s->stream);
         AVH_HIP(hipGetLastError());
@@ -485,6 +468,7 @@ static avh_status enqueue_step(avh_solver* s, EventMarks* tm = nullptr) {
         P.u0 = s->A;
         P.out = s->A;
         P.mode = MODE_RK2_2;
+        P.finish = 1;
         AVH_HIP(s->stage(P, s->shape, s->stream));
         if ((r = mark(s, tm))) return r;
     } else {
@@ -498,12 +482,10 @@ static avh_status enqueue_step(avh_solver* s, EventMarks* tm = nullptr) {
         P.u0 = s->A;
         P.out = s->A;
         P.mode = MODE_RK3_3;
+        P.finish = 1;
         AVH_HIP(s->stage(P, s->shape, s->stream));
         if ((r = mark(s, tm))) return r;
     }
-    hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, s->stream, s->ctrl);
-    AVH_HIP(hipGetLastError());
-    if ((r = mark(s, tm))) return r;
     return AVH_OK;
 }
 
@@ -888,12 +870,15 @@ avh_status avh_step_noalloc(avh_ctx* ctx, const avh_config* cfg, double* state, 
     if (cfg->integrator == AVH_INT_FORWARDEULER) {
         ew(B, state, state, ctrl, ncells, EW_FE_FINISH, cfg->gamma, st);
         AVH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, st, ctrl);
+        AVH_HIP(hipGetLastError());
     } else if (cfg->integrator == AVH_INT_RUNGEKUTTA2) {
         ew(B, state, C, ctrl, ncells, EW_RK2_U1, cfg->gamma, st);
         AVH_HIP(hipGetLastError());
         P.in = C;
         P.out = state;
         P.mode = MODE_RK2_2;
+        P.finish = 1;
         AVH_HIP(stage(P, shape, st));
     } else {
         P.in = B;
@@ -903,10 +888,9 @@ avh_status avh_step_noalloc(avh_ctx* ctx, const avh_config* cfg, double* state, 
         P.in = C;
         P.out = state;
         P.mode = MODE_RK3_3;
+        P.finish = 1;
         AVH_HIP(stage(P, shape, st));
     }
-    hipLaunchKernelGGL(finish_step_kernel, dim3(1), dim3(64), 0, st, ctrl);
-    AVH_HIP(hipGetLastError());
     hipLaunchKernelGGL(noalloc_result_kernel, dim3(1), dim3(64), 0, st, ctrl, dt_out, violated_out);
     AVH_HIP(hipGetLastError());
     return AVH_OK;
@@ -962,9 +946,10 @@ avh_status avh_time_step_kernels(avh_solver* s, int64_t steps, double* ms, int* 
     if (!s || !ms || !kernels || steps < 1 || steps > 100000) return fail(AVH_EINVAL, "bad argument");
     avh_status r = use_device(s);
     if (r) return r;
-    // launches per step: the stages (plus the element-wise pass of forward
-    // euler / rk2) and the end-of-step kernel
-    const int nk = s->cfg.integrator == AVH_INT_FORWARDEULER ? 3 : 4;
+    // launches per step: the stages, the element-wise pass of forward euler
+    // / rk2, and forward euler's end-of-step kernel (the rk final stages end
+    // the step themselves)
+    const int nk = 3;
     if ((r = ensure_shape(s))) return r;
     EventMarks t;
     if ((r = t.create((size_t)steps * (nk + 1)))) return r;

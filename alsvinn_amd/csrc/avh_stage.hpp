@@ -56,6 +56,10 @@ struct Control {
     int saves_pending;  // fixed-interval save times reached, not yet acknowledged
     int pause_on_save;
     int block_violation;
+    // workgroups of a finishing stage launch that are done (low 32 bits) and
+    // how many of them found a constraint violation (high 32 bits); the last
+    // one runs the end of the step and sets it back to 0
+    unsigned long long arrive;
 };
 
 enum StageMode {
@@ -79,6 +83,9 @@ struct StageParams {
     int bc;
     int mode;
     double gamma;
+    // 1: the launch is the step's last stage and its last workgroup runs the
+    // end of the step (end_step); 0: the caller runs finish_step_kernel
+    int finish = 0;
 };
 
 // The reference's Volume (alsfvm/src/volume/Volume.cpp:44-60): one array per
@@ -162,6 +169,49 @@ __device__ __forceinline__ double step_dt(const Control* c) {
         }
     }
     return dt;
+}
+
+// End of a step, one thread: t += dt, constraint flag, then the writers'
+// save counters (alsfvm/src/simulator/Simulator.cpp:81-85, 180-214;
+// alsfvm/src/io/FixedIntervalWriter.cpp:33-43) and the loop test
+// `t >= endTime` (Simulator.cpp:77-79); clears the wave-speed maxima for the
+// next step.
+__device__ __forceinline__ void end_step(Control* c, bool violation) {
+    const double dt = step_dt(c);
+    c->dt = dt;
+    c->scale = dt / c->cell[0];
+    c->t = c->t + dt;
+    c->steps = c->steps + 1;
+    c->last_dt = dt;
+    if (violation) {
+        c->violated = 1;
+    } else {
+        for (int i = 0; i < c->n_adj; ++i) {
+            if (c->t >= (double)c->adj_saved[i] * c->adj_interval[i] + c->adj_start[i]) {
+                c->adj_saved[i] = c->adj_saved[i] + 1;
+                c->saves_pending = c->saves_pending + 1;
+            }
+        }
+    }
+    c->block_violation = 0;
+    clear_wave_speeds(c);
+    const bool more = !(c->t >= c->t_end) && !c->violated && !(c->pause_on_save && c->saves_pending > 0);
+    c->active = more ? 1 : 0;
+}
+
+// Thread 0 of every workgroup of a step's last stage reports here once the
+// workgroup's cells are written (`bad`: a cell broke the constraints); the
+// last workgroup to arrive ends the step.  The count and the violations
+// travel in one atomic, so no fence is needed: the end of the step reads
+// only what earlier launches wrote (the first stage's wave speeds, the
+// previous step's bookkeeping).
+__device__ __forceinline__ void arrive_final(Control* c, bool bad) {
+    const unsigned long long add = 1ull + (bad ? (1ull << 32) : 0ull);
+    const unsigned long long prev = atomicAdd(&c->arrive, add);
+    if ((unsigned int)prev == gridDim.x - 1) {
+        c->arrive = 0ull;
+        end_step(c, ((prev + add) >> 32) != 0ull);
+    }
 }
 
 // MODE_CT >= 0 compiles the kernel for that StageMode alone (P.mode must
@@ -486,7 +536,11 @@ struct StageKernel {
                     atomicMax(&ctrl->ws_bits[threadIdx.x], (unsigned long long)__double_as_longlong(w));
             }
         } else if (final_stage) {
-            if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&ctrl->block_violation, 1);
+            const int any_bad = __syncthreads_or(bad);
+            if (threadIdx.x == 0) {
+                if (P.finish) arrive_final(ctrl, any_bad != 0);
+                else if (any_bad) atomicOr(&ctrl->block_violation, 1);
+            }
         }
     }
 };

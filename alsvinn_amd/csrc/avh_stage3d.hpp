@@ -585,7 +585,11 @@ struct MarchKernel3 {
                     atomicMax(&ctrl->ws_bits[threadIdx.x], (unsigned long long)__double_as_longlong(wv));
             }
         } else if (final_stage) {
-            if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&ctrl->block_violation, 1);
+            const int any_bad = __syncthreads_or(bad);
+            if (threadIdx.x == 0) {
+                if (P.finish) arrive_final(ctrl, any_bad != 0);
+                else if (any_bad) atomicOr(&ctrl->block_violation, 1);
+            }
         }
     }
 };

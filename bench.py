@@ -443,7 +443,8 @@ def main():
     alsvinn_amd._abi.check(L.avh_time_step_kernels(sim._h, args.profile_steps, ms, ctypes.byref(nk)))
     launch_ms = [ms[i] for i in range(nk.value)]
     stage_ms = launch_ms[:3]          # the three fused stage launches
-    finish_ms = launch_ms[3]          # one-wave end-of-step bookkeeping kernel
+    # RK3: the last stage's last workgroup ends the step (no end-of-step kernel)
+    finish_ms = launch_ms[3] if len(launch_ms) > 3 else None
     sb = stage_bytes_per_cell(NV)
     avg_ms = sum(stage_ms) / len(stage_ms)
     avg_bytes = cells * sum(sb) / len(sb)

@@ -201,8 +201,10 @@ avh_status avh_autotune(avh_solver* solver, int* best_shape, double* ms_per_shap
  * launch on the solver's stream (the hipEvent analogue of the reference's
  * ALSVINN_TIME_CUDA_BLOCK timers, alsutils/include/alsutils/timer/CudaTimer.hpp:11-48).
  * ms_per_launch[k] receives the average duration in milliseconds of the
- * k-th kernel of a step (k < *kernels_per_step <= 4), in launch order; the
- * last one is the end-of-step bookkeeping kernel. */
+ * k-th kernel of a step (k < *kernels_per_step <= 4), in launch order: the
+ * stages and the element-wise pass of forward euler / RK2, and forward
+ * euler's end-of-step kernel (the last RK2 / RK3 stage ends the step in its
+ * last workgroup). */
 avh_status avh_time_step_kernels(avh_solver* solver, int64_t steps, double* ms_per_launch,
                                  int* kernels_per_step);
 
