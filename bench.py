@@ -88,8 +88,14 @@ def valu_bound(prof, avg_ms, source):
     if not insts:
         return None
     peak = 1024 * 2.4e9 / 4
-    return {"insts_per_launch": insts, "achieved_per_s": insts / (avg_ms * 1e-3), "peak_per_s": peak,
-            "frac": insts / (avg_ms * 1e-3) / peak, "source": source}
+    out = {"insts_per_launch": insts, "achieved_per_s": insts / (avg_ms * 1e-3), "peak_per_s": peak,
+           "frac": insts / (avg_ms * 1e-3) / peak, "source": source}
+    busy = prof.get("stage_valu_busy_frac")
+    if busy:
+        # from the profile's own counter run: SQ_ACTIVE_INST_VALU (quarter-rate
+        # instructions weighted by four) over the SIMD time, per stage launch
+        out["pipe_busy_frac_profiled"] = busy
+    return out
 
 
 def host_cpu():
