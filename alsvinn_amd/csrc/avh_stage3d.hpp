@@ -159,19 +159,51 @@ struct MarchKernel3 {
         return (unsigned int)(((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + LZ - 1) / LZ));
     }
 
-    __device__ __forceinline__ static long long gidx(int x, int y, int z, int n0, int n1) {
-        return ((long long)z * n1 + y) * n0 + x;
+    // Cell indices.  The kernels compiled for the first two RK3 stages index
+    // with 32-bit values (their launcher takes them only for grids of fewer
+    // than 2^29 cells, so byte offsets stay below 2^32): each load is then a
+    // 32-bit offset from a variable's base held in scalar registers, with no
+    // 64-bit address arithmetic per lane.  (The final stage, which also
+    // stores, runs out of scalar registers that way and is slower.)
+    static constexpr bool OFF32 = MODE_CT == MODE_L || MODE_CT == MODE_RK3_2;
+    typedef typename std::conditional<OFF32, unsigned int, long long>::type Idx;
+    __device__ __forceinline__ static Idx gidx(int x, int y, int z, int n0, int n1) {
+        return ((Idx)z * n1 + y) * n0 + x;
+    }
+    typedef __attribute__((address_space(1))) char gchar;
+    // a uniform pointer as a scalar value the compiler does not re-associate
+    // with the lane offset
+    __device__ __forceinline__ static gchar* sbase(const double* p) {
+        const unsigned long long a = (unsigned long long)p;
+        const unsigned int lo = __builtin_amdgcn_readfirstlane((unsigned int)a);
+        const unsigned int hi = __builtin_amdgcn_readfirstlane((unsigned int)(a >> 32));
+        return (gchar*)(((unsigned long long)hi << 32) | lo);
+    }
+    // element v of cell gi of an SoA buffer of ncells cells per variable
+    __device__ __forceinline__ static double ld(const double* base, long long ncells, int v, Idx gi) {
+        if constexpr (OFF32) {
+            return *(const __attribute__((address_space(1))) double*)(sbase(base + v * ncells) + gi * 8u);
+        } else {
+            return base[v * ncells + gi];
+        }
+    }
+    __device__ __forceinline__ static void st(double* base, long long ncells, int v, Idx gi, double x) {
+        if constexpr (OFF32) {
+            *(__attribute__((address_space(1))) double*)(sbase(base + v * ncells) + gi * 8u) = x;
+        } else {
+            base[v * ncells + gi] = x;
+        }
     }
 
     // one input value at a (boundary-mapped) global cell; in the on-the-fly
     // stage U0's values are also written to `keep` (stride NC) when given
-    __device__ __forceinline__ static void fetch(const StageParams& P, bool otf, double s, long long gi, double* q,
+    __device__ __forceinline__ static void fetch(const StageParams& P, bool otf, double s, Idx gi, double* q,
                                                  double* keep = nullptr) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            double val = P.in[v * P.ncells + gi];
+            double val = ld(P.in, P.ncells, v, gi);
             if (otf) {
-                const double u = P.u0[v * P.ncells + gi];
+                const double u = ld(P.u0, P.ncells, v, gi);
                 if (keep) keep[v * NC] = u;
                 val = val * s + u;
             }
@@ -342,7 +374,7 @@ struct MarchKernel3 {
         // then the side columns.
         constexpr int HPT = (NHALO + NT - 1) / NT;
         int halo_lds[HPT];
-        long long halo_xy[HPT];
+        Idx halo_xy[HPT];
         bool has_halo[HPT];
 #pragma unroll
         for (int j = 0; j < HPT; ++j) {
@@ -370,10 +402,10 @@ struct MarchKernel3 {
                 gx = bc_map(gx, n0, P.bc);
                 gy = bc_map(gy, n1, P.bc);
                 halo_lds[j] = ly * LXP + lx;
-                halo_xy[j] = (long long)gy * n0 + gx;
+                halo_xy[j] = (Idx)gy * n0 + gx;
             }
         }
-        const long long plane = (long long)n0 * n1;
+        const Idx plane = (Idx)n0 * n1;
         // the ring of the next plane is loaded while the current plane is
         // computed: raw input (and U0 for the on-the-fly stage) in registers
         double pin[HPT][NV], pu0[HPT][NV];
@@ -384,10 +416,10 @@ struct MarchKernel3 {
 #define AVH_PREFETCH_RING(KP)                                                   \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) {                           \
         if (has_halo[j]) {                                                      \
-            const long long gi = (long long)(KP) * plane + halo_xy[j];          \
+            const Idx gi = (Idx)(KP) * plane + halo_xy[j];                      \
             _Pragma("unroll") for (int v = 0; v < NV; ++v) {                    \
-                pin[j][v] = P.in[v * P.ncells + gi];                            \
-                if (otf) pu0[j][v] = P.u0[v * P.ncells + gi];                   \
+                pin[j][v] = ld(P.in, P.ncells, v, gi);                            \
+                if (otf) pu0[j][v] = ld(P.u0, P.ncells, v, gi);                   \
             }                                                                   \
         }                                                                       \
     }
@@ -403,9 +435,9 @@ struct MarchKernel3 {
 #pragma unroll
         for (int v = 0; v < NV; ++v) u0n[v] = 0.0;
         if (combine && col_ok && !ring) {
-            const long long gi = gidx(ox + cx, oy + cy, z0, n0, n1);
+            const Idx gi = gidx(ox + cx, oy + cy, z0, n0, n1);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) u0n[v] = P.u0[v * P.ncells + gi];
+            for (int v = 0; v < NV; ++v) u0n[v] = ld(P.u0, P.ncells, v, gi);
         }
 
         for (int k = z0; k < z1; ++k) {
@@ -501,7 +533,7 @@ struct MarchKernel3 {
             for (int v = 0; v < NV; ++v) acc[v] = acc[v] + ((-fz[v]) + fprev[v]);
 
             if (col_ok) {
-                const long long gi = gidx(ox + cx, oy + cy, k, n0, n1);
+                const Idx gi = gidx(ox + cx, oy + cy, k, n0, n1);
                 if (ring) {
                     const double* const u0k = slot(k);
 #pragma unroll
@@ -524,7 +556,7 @@ struct MarchKernel3 {
                 }
                 if (final_stage) bad = bad || !EQ::obeys_fast(r, P.gamma);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) P.out[v * P.ncells + gi] = r[v];
+                for (int v = 0; v < NV; ++v) st(P.out, P.ncells, v, gi, r[v]);
             }
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -539,21 +571,21 @@ struct MarchKernel3 {
             // the window's next plane and the next write-back's U0, issued
             // together so that their latencies overlap
             if (has_col && k + 1 < z1) {
-                const long long gn = gidx(gxc, gyc, bc_map(k + 2 + HW, n2, P.bc), n0, n1);
-                const long long gw = gidx(ox + cx, oy + cy, k + 1, n0, n1);
+                const Idx gn = gidx(gxc, gyc, bc_map(k + 2 + HW, n2, P.bc), n0, n1);
+                const Idx gw = gidx(ox + cx, oy + cy, k + 1, n0, n1);
                 double cin[NV], cu0[NV];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    cin[v] = P.in[v * P.ncells + gn];
+                    cin[v] = ld(P.in, P.ncells, v, gn);
                     cu0[v] = 0.0;
                 }
                 if (otf) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) cu0[v] = P.u0[v * P.ncells + gn];
+                    for (int v = 0; v < NV; ++v) cu0[v] = ld(P.u0, P.ncells, v, gn);
                 }
                 if (combine && col_ok && !ring) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) u0n[v] = P.u0[v * P.ncells + gw];
+                    for (int v = 0; v < NV; ++v) u0n[v] = ld(P.u0, P.ncells, v, gw);
                 }
                 if (ring && k + 2 + HW < z1) {
                     double* const kp = slot(k + 2 + HW);

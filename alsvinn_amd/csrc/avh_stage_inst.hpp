@@ -147,6 +147,9 @@ hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
             // the RK3 stages of the shape a 3D run normally takes, each compiled
             // for its mode; other modes take the kernel that reads P.mode
             typedef typename std::conditional<SHAPE3_SHARED[6], FLUX, typename FLUX::Plain>::type F6;
+            if (P.ncells >= (1ll << 29))  // the per-mode kernels index with 32-bit byte offsets
+                return launch_march_k<MarchKernel3<EQ, RECON, F6, SHAPE3_TX[6], SHAPE3_TY[6], SHAPE3_LZ[6], 256,
+                                                   SHAPE3_TWO_BARRIERS[6]>>(P, stream);
             if (P.mode == MODE_L)
                 return launch_march_k<MarchKernel3<EQ, RECON, F6, SHAPE3_TX[6], SHAPE3_TY[6], SHAPE3_LZ[6], 256,
                                                    SHAPE3_TWO_BARRIERS[6], MODE_L>>(P, stream);
