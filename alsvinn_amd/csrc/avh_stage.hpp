@@ -107,6 +107,36 @@ __device__ __forceinline__ long long volume_index(const VolumeParams& V, int x, 
            (x + V.org[0] + V.ghost[0]);
 }
 
+// Element v of cell gi of an SoA buffer of nc cells per variable.  With
+// OFF32 the cell index is a 32-bit value (the caller guarantees nc < 2^29,
+// so byte offsets stay below 2^32) and the variable's base a uniform value
+// the compiler cannot re-associate with the lane offset: the access is then
+// a global load / store with a scalar base and a 32-bit lane offset, with no
+// 64-bit address arithmetic per lane.
+typedef __attribute__((address_space(1))) char global_char;
+__device__ __forceinline__ global_char* uniform_base(const double* p) {
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned int lo = __builtin_amdgcn_readfirstlane((unsigned int)a);
+    const unsigned int hi = __builtin_amdgcn_readfirstlane((unsigned int)(a >> 32));
+    return (global_char*)(((unsigned long long)hi << 32) | lo);
+}
+template <bool OFF32, class Idx>
+__device__ __forceinline__ double soa_ld(const double* base, long long nc, int v, Idx gi) {
+    if constexpr (OFF32) {
+        return *(const __attribute__((address_space(1))) double*)(uniform_base(base + v * nc) + gi * 8u);
+    } else {
+        return base[v * nc + gi];
+    }
+}
+template <bool OFF32, class Idx>
+__device__ __forceinline__ void soa_st(double* base, long long nc, int v, Idx gi, double x) {
+    if constexpr (OFF32) {
+        *(__attribute__((address_space(1))) double*)(uniform_base(base + v * nc) + gi * 8u) = x;
+    } else {
+        base[v * nc + gi] = x;
+    }
+}
+
 // Boundary map of a coordinate in [-G, n+G) onto the interior: periodic wrap
 // (alsfvm/include/alsfvm/boundary/Periodic.hpp:24-42) or neumann mirror
 // (alsfvm/include/alsfvm/boundary/Neumann.hpp:24-35).  Applied per axis this
@@ -218,6 +248,10 @@ __device__ __forceinline__ void arrive_final(Control* c, bool bad) {
 // equal it): the other modes' branches and registers drop out.
 template <class EQ, int RECON, class FLUX, int DIM, int TX, int TY, int TZ, int NT, int MODE_CT = -1>
 struct StageKernel {
+    // the kernels compiled per stage mode index cells with 32-bit values
+    // (their launchers take them only for grids of fewer than 2^29 cells)
+    static constexpr bool OFF32 = MODE_CT >= 0;
+    typedef typename std::conditional<OFF32, unsigned int, long long>::type Idx;
     static constexpr int NV = EQ::NV;
     static constexpr int G = ReconTraits<RECON>::GHOST;
     static constexpr int HX = G;
@@ -387,7 +421,7 @@ struct StageKernel {
         // them, so that their global-memory latencies overlap; points past
         // the tile repeat the last one and are not written.
         constexpr int SPT = (LT + NT - 1) / NT;  // staged points per thread
-        long long sg[SPT];
+        Idx sg[SPT];
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
             int p = threadIdx.x + k * NT;
@@ -405,7 +439,7 @@ struct StageKernel {
                 gx = bc_map(gx, n0, P.bc);
                 gy = DIM >= 2 ? bc_map(gy, n1, P.bc) : 0;
                 gz = DIM >= 3 ? bc_map(gz, n2, P.bc) : 0;
-                sg[k] = ((long long)gz * n1 + gy) * n0 + gx;
+                sg[k] = ((Idx)gz * n1 + gy) * n0 + gx;
             }
         }
         double sv[SPT][NV];
@@ -414,13 +448,13 @@ struct StageKernel {
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 if constexpr (VOL) sv[k][v] = V->in[v][sg[k]];
-                else sv[k][v] = P.in[v * nc + sg[k]];
+                else sv[k][v] = soa_ld<OFF32>(P.in, nc, v, sg[k]);
             }
         if (otf) {
 #pragma unroll
             for (int k = 0; k < SPT; ++k)
 #pragma unroll
-                for (int v = 0; v < NV; ++v) sv[k][v] = sv[k][v] * s + P.u0[v * nc + sg[k]];
+                for (int v = 0; v < NV; ++v) sv[k][v] = sv[k][v] * s + soa_ld<OFF32>(P.u0, nc, v, sg[k]);
         }
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
@@ -461,7 +495,7 @@ struct StageKernel {
         bool bad = false;
         const bool final_stage = (mode == MODE_RK2_2 || mode == MODE_RK3_3);
         const bool combine = !(mode == MODE_FLUX_ONLY || mode == MODE_L);
-        long long gc[CPT];
+        Idx gc[CPT];
         bool inside[CPT];
 #pragma unroll
         for (int m = 0; m < CPT; ++m) {
@@ -472,7 +506,7 @@ struct StageKernel {
             const int gx = ox + cx, gy = oy + cy, gz = oz + cz;
             inside[m] = (TC % NT == 0 || c < TC) && gx < n0 && gy < n1 && gz < n2;
             if constexpr (VOL) gc[m] = inside[m] ? volume_index(*V, gx, gy, gz) : 0;
-            else gc[m] = inside[m] ? ((long long)gz * n1 + gy) * n0 + gx : 0;
+            else gc[m] = inside[m] ? ((Idx)gz * n1 + gy) * n0 + gx : 0;
         }
         const double ca = mode == MODE_RK3_2 ? 1. / 3. : (mode == MODE_RK3_3 ? 2.0 : 1.0);
         const double cb = mode == MODE_RK2_2 ? 0.5 : (mode == MODE_RK3_2 ? 3. / 4. : 1. / 3.);
@@ -481,7 +515,7 @@ struct StageKernel {
 #pragma unroll
             for (int m = 0; m < CPT; ++m)
 #pragma unroll
-                for (int v = 0; v < NV; ++v) u0c[m][v] = P.u0[v * nc + gc[m]];
+                for (int v = 0; v < NV; ++v) u0c[m][v] = soa_ld<OFF32>(P.u0, nc, v, gc[m]);
         }
 #pragma unroll
         for (int m = 0; m < CPT; ++m) {
@@ -490,7 +524,7 @@ struct StageKernel {
             const int cx = c % TX;
             const int cy = (c / TX) % TY;
             const int cz = c / (TX * TY);
-            const long long gi = gc[m];
+            const Idx gi = gc[m];
             const int li = lidx(cx, cy, cz);
             double r[NV];
 #pragma unroll
@@ -514,7 +548,7 @@ struct StageKernel {
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 if constexpr (VOL) V->out[v][gi] = r[v];
-                else P.out[v * nc + gi] = r[v];
+                else soa_st<OFF32>(P.out, nc, v, gi, r[v]);
             }
         }
 

@@ -170,29 +170,11 @@ struct MarchKernel3 {
     __device__ __forceinline__ static Idx gidx(int x, int y, int z, int n0, int n1) {
         return ((Idx)z * n1 + y) * n0 + x;
     }
-    typedef __attribute__((address_space(1))) char gchar;
-    // a uniform pointer as a scalar value the compiler does not re-associate
-    // with the lane offset
-    __device__ __forceinline__ static gchar* sbase(const double* p) {
-        const unsigned long long a = (unsigned long long)p;
-        const unsigned int lo = __builtin_amdgcn_readfirstlane((unsigned int)a);
-        const unsigned int hi = __builtin_amdgcn_readfirstlane((unsigned int)(a >> 32));
-        return (gchar*)(((unsigned long long)hi << 32) | lo);
-    }
-    // element v of cell gi of an SoA buffer of ncells cells per variable
     __device__ __forceinline__ static double ld(const double* base, long long ncells, int v, Idx gi) {
-        if constexpr (OFF32) {
-            return *(const __attribute__((address_space(1))) double*)(sbase(base + v * ncells) + gi * 8u);
-        } else {
-            return base[v * ncells + gi];
-        }
+        return soa_ld<OFF32>(base, ncells, v, gi);
     }
     __device__ __forceinline__ static void st(double* base, long long ncells, int v, Idx gi, double x) {
-        if constexpr (OFF32) {
-            *(__attribute__((address_space(1))) double*)(sbase(base + v * ncells) + gi * 8u) = x;
-        } else {
-            base[v * ncells + gi] = x;
-        }
+        soa_st<OFF32>(base, ncells, v, gi, x);
     }
 
     // one input value at a (boundary-mapped) global cell; in the on-the-fly

@@ -56,8 +56,11 @@ struct KernelFor<EQ, RECON, FLUX, 2, S> {
     typedef StageKernel<EQ, RECON, FLUX, 2, SHAPE2_TX[S], SHAPE2_TY[S], 1, 256> type;
 };
 // 2D shapes whose RK3 stages are each compiled for their mode (the shapes
-// the autotuner takes for the benchmark grids and the batched set)
+// the autotuner takes for the benchmark grids and the batched set); those
+// kernels index cells with 32-bit values, so they serve grids of fewer than
+// 2^29 cells (per sample in a batch)
 constexpr bool SHAPE2_BY_MODE[NSHAPES2] = {false, false, false, true, true};
+constexpr long long MODE_KERNEL_MAX_CELLS = 1ll << 29;
 template <class EQ, int RECON, class FLUX, int S, int MODE>
 struct KernelForMode {
     typedef StageKernel<EQ, RECON, FLUX, 2, SHAPE2_TX[S], SHAPE2_TY[S], 1, 256, MODE> type;
@@ -90,17 +93,20 @@ hipError_t launch_batch_stage(const BatchParams& B, unsigned int nblocks, int sh
 #define AVH_BATCH_MODES(S)                                                                                  \
     case S:                                                                                                 \
         if constexpr (SHAPE2_BY_MODE[S]) {                                                                  \
-            if (B.mode == MODE_L)                                                                           \
+            if (!small) {                                                                                   \
+            } else if (B.mode == MODE_L)                                                                    \
                 return launch_batch_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_L>::type>(B, nblocks, \
                                                                                                stream);     \
-            if (B.mode == MODE_RK3_2)                                                                       \
+            else if (B.mode == MODE_RK3_2)                                                                  \
                 return launch_batch_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_RK3_2>::type>(        \
                     B, nblocks, stream);                                                                    \
-            if (B.mode == MODE_RK3_3)                                                                       \
+            else if (B.mode == MODE_RK3_3)                                                                  \
                 return launch_batch_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_RK3_3>::type>(        \
                     B, nblocks, stream);                                                                    \
         }                                                                                                   \
         return launch_batch_k<typename KernelFor<EQ, RECON, FLUX, 2, S>::type>(B, nblocks, stream);
+        bool small = true;
+        for (int i = 0; i < B.ngroups; ++i) small = small && B.g[i].ncells < MODE_KERNEL_MAX_CELLS;
         switch (shape) {
             AVH_BATCH_MODES(0)
             AVH_BATCH_MODES(1)
@@ -147,7 +153,7 @@ hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
             // the RK3 stages of the shape a 3D run normally takes, each compiled
             // for its mode; other modes take the kernel that reads P.mode
             typedef typename std::conditional<SHAPE3_SHARED[6], FLUX, typename FLUX::Plain>::type F6;
-            if (P.ncells >= (1ll << 29))  // the per-mode kernels index with 32-bit byte offsets
+            if (P.ncells >= MODE_KERNEL_MAX_CELLS)  // the per-mode kernels index with 32-bit values
                 return launch_march_k<MarchKernel3<EQ, RECON, F6, SHAPE3_TX[6], SHAPE3_TY[6], SHAPE3_LZ[6], 256,
                                                    SHAPE3_TWO_BARRIERS[6]>>(P, stream);
             if (P.mode == MODE_L)
@@ -175,12 +181,13 @@ hipError_t launch_stage(const StageParams& P, int shape, hipStream_t stream) {
 #define AVH_STAGE_MODES(S)                                                                                  \
     case S:                                                                                                 \
         if constexpr (SHAPE2_BY_MODE[S]) {                                                                  \
-            if (P.mode == MODE_L)                                                                           \
+            if (P.ncells >= MODE_KERNEL_MAX_CELLS) {                                                        \
+            } else if (P.mode == MODE_L)                                                                    \
                 return launch_stage_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_L>::type>(P, stream); \
-            if (P.mode == MODE_RK3_2)                                                                       \
+            else if (P.mode == MODE_RK3_2)                                                                  \
                 return launch_stage_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_RK3_2>::type>(P,      \
                                                                                                  stream);   \
-            if (P.mode == MODE_RK3_3)                                                                       \
+            else if (P.mode == MODE_RK3_3)                                                                  \
                 return launch_stage_k<typename KernelForMode<EQ, RECON, FLUX, S, MODE_RK3_3>::type>(P,      \
                                                                                                  stream);   \
         }                                                                                                   \
