@@ -394,8 +394,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path to run; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.steps is None:
